@@ -1,0 +1,104 @@
+/*
+ * sq_jk.h - C-ABI of the MI355X two-electron engine.
+ *
+ * Two layers:
+ *  1. Drop-in entry points with the exact signatures of the reference's J/K
+ *     surface, so a caller of the reference links against this library
+ *     unchanged:
+ *       GTO_JK_Matrix_Quartet_Parallel  <- src/matrix.h:155-166 (src/matrix.c:3773-3986)
+ *       uhf_getGMatrix                  <- src/uhf.h:26-33      (src/uhf.c:112-202)
+ *       create_Schwarz                  <- src/matrix.h:69-71   (src/matrix.c:389-440)
+ *  2. A re-entrant context API (sqjk_*) that the drop-ins are built on.  It
+ *     holds per-molecule state explicitly instead of in statics, so several
+ *     molecules can be served side by side.
+ *
+ * All matrices are row-major n x n double arrays in host memory.  Densities
+ * are assumed symmetric (as in the reference).  Every call runs the integral
+ * and digestion work on the HIP device; there is no CPU fallback: when no
+ * device is usable the drop-ins print an error and exit(-1), matching the
+ * reference's error behaviour, and the sqjk_* calls return SQJK_ENODEV.
+ */
+#ifndef SQ_JK_H
+#define SQ_JK_H
+
+#include "sq_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define SQJK_OK        0
+#define SQJK_ENODEV   -1  /* no usable HIP device                        */
+#define SQJK_EHIP     -2  /* a HIP runtime call failed                    */
+#define SQJK_EBASIS   -3  /* basis not representable (see sqjk_errmsg)   */
+#define SQJK_EARG     -4  /* invalid argument                             */
+#define SQJK_ENOMEM   -5  /* device allocation failed                     */
+
+typedef struct sqjk_ctx sqjk_ctx;
+
+/* Per-build statistics, filled by sqjk_build / sqjk_last_stats. */
+typedef struct {
+	double kernel_ms;      /* integral+digestion kernels, device time      */
+	double total_ms;       /* whole build incl. uploads/downloads (host)   */
+	long long quartets;    /* contracted shell quartets launched           */
+	long long quartets_eval; /* quartets surviving density screening (0 if not counted) */
+	int launches;          /* kernel launches in this build                */
+} sqjk_stats;
+
+/* Create a context for one basis.  schwarz_basis: optional n x n matrix of
+ * sqrt|(pq|pq)| (as from create_Schwarz); NULL computes it on the device.
+ * device: HIP device ordinal.  Returns SQJK_OK and *out on success. */
+int sqjk_create(int nBasis, const struct GTOBasis_t *gto, const double *schwarz_basis,
+                int device, sqjk_ctx **out);
+
+/* One J/K build on density PA (alpha) and PB (beta):
+ *   JT += J[PA+PB],  KA -= K[PA],  KB -= K[PB]   (accumulating, as the
+ * reference does), then KB := KA when restricted, then upper triangle :=
+ * lower triangle of JT, KA, KB.
+ * Screening follows the reference: shell quartets with
+ * Sch_PQ*Sch_IJ < max(scfCutoff, fixedCutoff), or with
+ * Sch_PQ*Sch_IJ*(sum of six block-max |P|) < fixedCutoff, are skipped, and
+ * primitive quartets with a bare prefactor below 1e-15 are dropped. */
+int sqjk_build(sqjk_ctx *ctx, const double *PA, const double *PB, double fixedCutoff,
+               double scfCutoff, int restricted, double *JT, double *KA, double *KB);
+
+/* Schwarz matrix sqrt|(pq|pq)| (n x n) as held by the context. */
+int sqjk_schwarz(sqjk_ctx *ctx, double *out);
+
+/* Statistics of the last build. */
+int sqjk_last_stats(const sqjk_ctx *ctx, sqjk_stats *st);
+
+/* Number of basis functions, reference shells and pure-l sub-shells. */
+int sqjk_dims(const sqjk_ctx *ctx, int *nBasis, int *nShell, int *nSubShell);
+
+void sqjk_destroy(sqjk_ctx *ctx);
+
+/* Text of the last error on this thread. */
+const char *sqjk_errmsg(void);
+
+/* ---- drop-in surface (reference signatures) ---- */
+
+/* src/matrix.h:155-166.  Same accumulate / sign / restricted / symmetrise
+ * semantics; the per-molecule state lives in one library-held context that
+ * is built on the first call and released by a call with nBasis == 0, as in
+ * the reference (src/matrix.c:3817-3828).  With opt->nCPU > 1 all work is
+ * done by childID 0 and other childIDs add nothing, so the caller's sum over
+ * children is unchanged.  Reads opt->SCFCutoff, opt->restricted, opt->nCPU. */
+void GTO_JK_Matrix_Quartet_Parallel(int childID, int nBasis, const double *PA, const double *PB,
+                                    const struct GTOBasis_t *gto, const double *schwarz_basis,
+                                    double fixedCutoff, double *JT, double *KA, double *KB,
+                                    struct option_t *opt);
+
+/* src/uhf.h:26-33: zero JT, KA, KB, then one J/K build. */
+void uhf_getGMatrix(int nBasis, const struct GTOBasis_t *gto, const double *Schwarz,
+                    double cutoff, const double *PA, const double *PB,
+                    double *JT, double *KA, double *KB, struct option_t *opt);
+
+/* src/matrix.h:69-71: malloc'd n x n Schwarz matrix, computed on the device. */
+double *create_Schwarz(int nBasis, const struct GTOBasis_t *gto);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SQ_JK_H */

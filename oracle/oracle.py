@@ -1,0 +1,189 @@
+"""ctypes wrapper of the oracle (test infrastructure only).
+
+`Oracle` drives liboracle.so (the C restatement in sq_oracle.c);
+`Reference` drives _ref/libsqref.so, the reference's own hot-path files
+compiled by oracle/Makefile.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py use this module.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "siam-quantum_amd"))
+from sqamd.abi import (GTOBasis, GTOBasisSet, Molecule, Option, Basis, Mol,  # noqa: E402
+                       basis_from_ptr, default_option, dptr)
+
+D, I, P = C.c_double, C.c_int, C.c_void_p
+PD = C.POINTER(C.c_double)
+PG = C.POINTER(GTOBasis)
+PM = C.POINTER(Molecule)
+
+
+def _lib(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+    return C.CDLL(path)
+
+
+class Oracle:
+    def __init__(self):
+        L = self.L = _lib(os.path.join(HERE, "liboracle.so"))
+        L.or_fgamma_set.argtypes = [I, D, PD]
+        L.or_fgamma.argtypes = [I, D]
+        L.or_fgamma.restype = D
+        L.or_eri.argtypes = [D, D, D, D, I, I, I, D] * 4
+        L.or_eri.restype = D
+        L.or_overlap.argtypes = [D, I, I, I, D, D, D] * 2
+        L.or_overlap.restype = D
+        L.or_kinetic.argtypes = [D, I, I, I, D, D, D] * 2
+        L.or_kinetic.restype = D
+        L.or_nai.argtypes = [D, D, D, D, I, I, I, D] * 2 + [D, D, D]
+        L.or_nai.restype = D
+        L.or_contr_eri.argtypes = [PG, I, I, I, I]
+        L.or_contr_eri.restype = D
+        L.or_schwarz.argtypes = [I, PG, PD]
+        L.or_gto_overlap.argtypes = [I, I, PG]
+        L.or_gto_overlap.restype = D
+        L.or_gto_kinetic.argtypes = [I, I, PG]
+        L.or_gto_kinetic.restype = D
+        L.or_gto_nuclei.argtypes = [I, I, PG, PM]
+        L.or_gto_nuclei.restype = D
+        L.or_jk_create.argtypes = [I, I]
+        L.or_jk_create.restype = P
+        L.or_jk_reset.argtypes = [P]
+        L.or_jk_destroy.argtypes = [P]
+        L.or_jk_build.argtypes = [P, I, PD, PD, PG, PD, D, D, I, PD, PD, PD]
+        L.or_jk_dense.argtypes = [I, PG, PD, PD, PD, PD, PD]
+        L.or_read_xyz.argtypes = [C.c_char_p]
+        L.or_read_xyz.restype = PM
+        L.or_read_basis.argtypes = [C.c_char_p, C.POINTER(I)]
+        L.or_read_basis.restype = C.POINTER(GTOBasisSet)
+        L.or_gen_basis.argtypes = [PM, I, C.POINTER(GTOBasisSet), C.POINTER(I)]
+        L.or_gen_basis.restype = PG
+        L.or_nuclei_coulomb.argtypes = [PM]
+        L.or_nuclei_coulomb.restype = D
+        L.or_gen_eigen.argtypes = [I, PD, PD, PD, PD]
+        L.or_gen_eigen.restype = I
+        L.or_uhf.argtypes = [I, PG, PM, I, I, I, D, I, D, I, I, I, I, PD, PD, PD, PD, C.POINTER(I)]
+        L.or_uhf.restype = D
+
+    # -- scalar pieces -------------------------------------------------
+    def boys(self, mmax, T):
+        F = np.zeros(mmax + 1)
+        self.L.or_fgamma_set(mmax, T, dptr(F))
+        return F
+
+    # -- inputs --------------------------------------------------------
+    def molecule(self, xyz_path, charge=0):
+        m = self.L.or_read_xyz(xyz_path.encode())
+        if not m:
+            raise IOError(xyz_path)
+        m.contents.Q = charge
+        return m
+
+    def basis(self, mol, basis_path):
+        n = I()
+        db = self.L.or_read_basis(basis_path.encode(), C.byref(n))
+        if not db:
+            raise IOError(basis_path)
+        nb = I()
+        g = self.L.or_gen_basis(mol, n.value, db, C.byref(nb))
+        if nb.value <= 0:
+            raise ValueError(f"element Z={-nb.value} missing from {basis_path}")
+        return g, nb.value
+
+    # -- matrices ------------------------------------------------------
+    def schwarz(self, g, n):
+        s = np.zeros((n, n))
+        self.L.or_schwarz(n, g, dptr(s))
+        return s
+
+    def one_electron(self, g, n, mol):
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        for i in range(n):
+            for j in range(i + 1):
+                S[i, j] = S[j, i] = self.L.or_gto_overlap(i, j, g)
+                T[i, j] = T[j, i] = self.L.or_gto_kinetic(i, j, g)
+                V[i, j] = V[j, i] = self.L.or_gto_nuclei(i, j, g, mol)
+        return S, T, V
+
+    def jk_context(self, use_mpole=False, max_mem=0):
+        return self.L.or_jk_create(int(use_mpole), int(max_mem))
+
+    def jk(self, ctx, g, n, PA, PB, sch, fixed_cutoff, scf_cutoff, restricted):
+        """One call of the restated J/K driver: returns (JT, KA, KB) increments."""
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = np.ascontiguousarray(PB, dtype=np.float64)
+        sch = np.ascontiguousarray(sch, dtype=np.float64)
+        J, KA, KB = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self.L.or_jk_build(ctx, n, dptr(PA), dptr(PB), g, dptr(sch), fixed_cutoff, scf_cutoff,
+                           int(restricted), dptr(J), dptr(KA), dptr(KB))
+        return J, KA, KB
+
+    def jk_dense(self, g, n, PA, PB):
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = np.ascontiguousarray(PB, dtype=np.float64)
+        J, KA, KB = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self.L.or_jk_dense(n, g, dptr(PA), dptr(PB), dptr(J), dptr(KA), dptr(KB))
+        return J, KA, KB
+
+    def uhf(self, g, n, mol, nEA, nEB, restricted=True, scf_conv=1e-8, scf_max=80, drag=0.25,
+            three_step=True, guess=0, use_mpole=True, max_mem=250):
+        CA, CB = np.zeros((n, n)), np.zeros((n, n))
+        eA, eB = np.zeros(n), np.zeros(n)
+        it = I()
+        E = self.L.or_uhf(n, g, mol, nEA, nEB, int(restricted), scf_conv, scf_max, drag,
+                          int(three_step), guess, int(use_mpole), max_mem,
+                          dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(it))
+        return E, it.value
+
+
+class Reference:
+    """The reference's own code (oracle/_ref/libsqref.so), used only to pin
+    the oracle and to time the CPU baseline."""
+
+    def __init__(self):
+        L = self.L = _lib(os.path.join(HERE, "_ref", "libsqref.so"))
+        L.fgamma_set.argtypes = [I, D, PD]
+        L.eri.argtypes = [D, D, D, D, I, I, I, D] * 4
+        L.eri.restype = D
+        L.create_Schwarz.argtypes = [I, PG]
+        L.create_Schwarz.restype = PD
+        L.GTO_JK_Matrix_Quartet_Parallel.argtypes = [I, I, PD, PD, PG, PD, D, PD, PD, PD, C.POINTER(Option)]
+        L.GTO_overlap.argtypes = [I, I, PG]
+        L.GTO_overlap.restype = D
+        L.GTO_kinetic.argtypes = [I, I, PG]
+        L.GTO_kinetic.restype = D
+        L.GTO_nuclei.argtypes = [I, I, PG, PM]
+        L.GTO_nuclei.restype = D
+        self.libc = C.CDLL(None)
+        self.libc.free.argtypes = [P]
+
+    def boys(self, mmax, T):
+        F = np.zeros(mmax + 1)
+        self.L.fgamma_set(mmax, T, dptr(F))
+        return F
+
+    def schwarz(self, g, n):
+        p = self.L.create_Schwarz(n, g)
+        s = np.ctypeslib.as_array(p, (n, n)).copy()
+        self.libc.free(C.cast(p, P))
+        return s
+
+    def jk(self, g, n, PA, PB, sch, fixed_cutoff, opt):
+        """GTO_JK_Matrix_Quartet_Parallel(childID=0) on zeroed outputs."""
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = np.ascontiguousarray(PB, dtype=np.float64)
+        sch = np.ascontiguousarray(sch, dtype=np.float64)
+        J, KA, KB = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self.L.GTO_JK_Matrix_Quartet_Parallel(0, n, dptr(PA), dptr(PB), g, dptr(sch), fixed_cutoff,
+                                              dptr(J), dptr(KA), dptr(KB), C.byref(opt))
+        return J, KA, KB
+
+    def jk_reset(self, opt):
+        z = np.zeros(1)
+        self.L.GTO_JK_Matrix_Quartet_Parallel(0, 0, dptr(z), dptr(z), None, dptr(z), 0.0,
+                                              dptr(z), dptr(z), dptr(z), C.byref(opt))

@@ -1,0 +1,64 @@
+// sq_jk_class.hip - kernels and launcher of one angular class, selected at
+// compile time with -DSQ_LA=.. -DSQ_LB=.. -DSQ_LC=.. -DSQ_LD=.. (the build
+// compiles this file once per class, in parallel).
+#include "sq_jk_classes.h"
+#include "sq_jk_wave.h"
+
+#if !defined(SQ_LA) || !defined(SQ_LB) || !defined(SQ_LC) || !defined(SQ_LD)
+#error "define SQ_LA SQ_LB SQ_LC SQ_LD"
+#endif
+
+#define SQ_CAT2(a, b, c, d, e) a##b##c##d##e
+#define SQ_CAT(a, b, c, d, e) SQ_CAT2(a, b, c, d, e)
+#define SQ_LAUNCHER SQ_CAT(sqjk_launch_, SQ_LA, SQ_LB, SQ_LC, SQ_LD)
+
+namespace {
+using namespace sqjk;
+constexpr int LA = SQ_LA, LB = SQ_LB, LC = SQ_LC, LD = SQ_LD;
+using SI = SliceInfo<LA, LB, LC, LD>;
+
+template <int MODE, int S>
+int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
+  if constexpr (S >= SI::NSLICE || SI::NAB * SI::NCD > 81) {
+    return 0;
+  } else {
+    constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
+    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S>), dim3(grid), dim3(256), lds, st, a);
+    if (hipGetLastError() != hipSuccess) return -1;
+    const int rest = launch_from<MODE, S + 1>(a, grid, st);
+    return rest < 0 ? -1 : rest + 1;
+  }
+}
+}  // namespace
+
+// Classes with more than 81 integrals per quartet use the cooperative
+// one-wavefront-per-quartet kernel; the others the per-thread kernel.
+constexpr bool kWave = SI::NAB * SI::NCD > 81;
+
+template <int MODE>
+int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((k_class_wave<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+template <int MODE>
+int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
+  if constexpr (kWave) {
+    // grid counts 256-thread blocks of the per-thread kernel; the wave
+    // kernel uses one 64-thread workgroup per quartet in flight
+    return launch_wave<MODE>(a, grid * 4, st);
+  } else {
+    return launch_from<MODE, 0>(a, grid, st);
+  }
+}
+
+int SQ_LAUNCHER(const sqjk::ClassArgs& a, int mode, int grid, hipStream_t st) {
+  switch (mode) {
+    case sqjk::kRHF: return launch_mode<sqjk::kRHF>(a, grid, st);
+    case sqjk::kUHF: return launch_mode<sqjk::kUHF>(a, grid, st);
+#if SQ_LA == SQ_LC && SQ_LB == SQ_LD
+    case sqjk::kSchwarz: return launch_mode<sqjk::kSchwarz>(a, grid, st);
+#endif
+    default: return -1;
+  }
+}

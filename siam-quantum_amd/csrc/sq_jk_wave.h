@@ -1,0 +1,314 @@
+// sq_jk_wave.h - cooperative kernel for the high-angular-momentum classes.
+//
+// One 64-lane workgroup (one wavefront) owns one contracted shell quartet at
+// a time.  Per primitive quartet the lanes build, in LDS,
+//   R_{TUV}   (closed form, one entry per lane),
+//   E^{ab}, E^{cd} Hermite coefficients (one axis per lane),
+//   W[cd][tuv] = sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V},
+// and every lane accumulates its share of the (ab|cd) block in registers.
+// The contracted block is then staged in LDS and digested into J/K with one
+// atomic per J/K element touched by the quartet.
+#pragma once
+#include "sq_jk_dev.h"
+
+namespace sqjk {
+
+// (t,u,v) of the compact Hermite index, up to total order 8
+struct HermTab {
+  unsigned char t[nherm(8)], u[nherm(8)], v[nherm(8)];
+};
+__host__ __device__ constexpr HermTab make_herm_tab() {
+  HermTab h{};
+  for (int n = 0; n <= 8; ++n)
+    for (int m = 0; m <= n; ++m)
+      for (int v = 0; v <= m; ++v) {
+        const int u = m - v, t = n - m;
+        const int i = hidx(t, u, v);
+        h.t[i] = (unsigned char)t; h.u[i] = (unsigned char)u; h.v[i] = (unsigned char)v;
+      }
+  return h;
+}
+__constant__ constexpr HermTab kHerm = make_herm_tab();
+
+// d^T/dX^T of g(X^2):  sum_i c(T,i) (2X)^(T-2i) g^(T-i),  c = T!/(i!(T-2i)!)
+__constant__ constexpr double kHermC[9][5] = {
+    {1, 0, 0, 0, 0},   {1, 0, 0, 0, 0},   {1, 2, 0, 0, 0},    {1, 6, 0, 0, 0},     {1, 12, 12, 0, 0},
+    {1, 20, 60, 0, 0}, {1, 30, 180, 120, 0}, {1, 42, 420, 840, 0}, {1, 56, 840, 3360, 1680}};
+
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // 64-thread workgroups
+
+// F_n(T) for one n straight from the global table (Taylor about the grid point)
+__device__ __forceinline__ double boys_one(int n, double T, const double* __restrict__ tab) {
+  if (T < kBoysTMax - 0.5 * kBoysStep) {
+    const int k = (int)(T * (1.0 / kBoysStep) + 0.5);
+    const double d = k * kBoysStep - T;
+    const double* row = tab + k * kBoysNM + n;
+    double s = row[kBoysTaylor - 1];
+#pragma unroll
+    for (int j = kBoysTaylor - 2; j >= 0; --j) s = s * (d * (1.0 / (j + 1))) + row[j];
+    return s;
+  }
+  // asymptotic: (2n-1)!!/(2T)^n * sqrt(pi/T)/2 minus a term below 1e-17 here
+  double f = 0.88622692545275801365 * sqrt(1.0 / T);
+  const double e = exp(-T), h = 0.5 / T;
+  for (int m = 0; m < n; ++m) f = ((2 * m + 1) * f - e) * h;
+  return f;
+}
+
+template <int LI, int LJ>
+__device__ __forceinline__ void herm_axis_to_lds(double* out, double xpa, double xpb, double h2p) {
+  Herm1<LI, LJ> E;
+  herm_coef<LI, LJ>(E, xpa, xpb, h2p);
+  sfor<LI + 1>([&](auto i) {
+    sfor<LJ + 1>([&](auto j) {
+      sfor<LI + LJ + 1>([&](auto t) {
+        out[(i.value * (LJ + 1) + j.value) * (LI + LJ + 1) + t.value] =
+            (t.value <= i.value + j.value) ? E.e[i.value][j.value][t.value] : 0.0;
+      });
+    });
+  });
+}
+
+template <int LA, int LB, int LC, int LD>
+struct WaveLayout {
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  static constexpr int NAB = NA * NB, NCD = NC * ND, NINT = NAB * NCD;
+  static constexpr int NH = nherm(L), NHAB = nherm(LAB);
+  static constexpr int EA = (LA + 1) * (LB + 1) * (LAB + 1);  // per axis
+  static constexpr int EC = (LC + 1) * (LD + 1) * (LCD + 1);
+  static constexpr int NW = NCD * NHAB;
+  static constexpr int NBIG = NW > NINT ? NW : NINT;
+  // LDS offsets (doubles)
+  static constexpr int oR = 0, oEa = oR + NH, oEc = oEa + 3 * EA, oW = oEc + 3 * EC;
+  static constexpr int oD = oW + NBIG;                 // staged density blocks
+  static constexpr int ND6 = NAB + NCD + NB * ND + NA * ND + NB * NC + NA * NC;
+  static constexpr int TOTAL = oD + 2 * ND6;
+  static constexpr int NOUT = (NINT + 63) / 64;         // integrals per lane
+};
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
+  using W_ = WaveLayout<LA, LB, LC, LD>;
+  constexpr int NA = W_::NA, NB = W_::NB, NC = W_::NC, ND = W_::ND;
+  constexpr int NAB = W_::NAB, NCD = W_::NCD, NINT = W_::NINT, L = W_::L, LAB = W_::LAB;
+  __shared__ double lds[W_::TOTAL];
+  double* R = lds + W_::oR;
+  double* Ea = lds + W_::oEa;
+  double* Ec = lds + W_::oEc;
+  double* Wt = lds + W_::oW;
+  const int lane = threadIdx.x;
+  const int n = A.n;
+
+  for (long long t = blockIdx.x; t < A.total; t += gridDim.x) {
+    int ib, ik;
+    if constexpr (MODE == kSchwarz) {
+      ib = ik = (int)t;
+    } else {
+      ib = find_bra(A.off, A.nbra, t);
+      ik = (int)(t - A.off[ib]);
+      const PairRec bp = A.bra[ib], kp = A.ket[ik];
+      const double* PM = A.PM;
+      const int nr = A.nref;
+      const double mp = PM[bp.pa * nr + bp.pb] + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
+                        PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
+      if (bp.sch * kp.sch * mp < A.fixed_cut) continue;   // uniform in the workgroup
+    }
+    const PairRec bp = A.bra[ib], kp = A.ket[ik];
+    const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
+
+    double acc[W_::NOUT];
+#pragma unroll
+    for (int k = 0; k < W_::NOUT; ++k) acc[k] = 0.0;
+
+    for (int u = 0; u < bp.npp; ++u) {
+      const PrimPair P = A.pp[bp.pp_off + u];
+      for (int v = 0; v < kp.npp; ++v) {
+        const PrimPair Q = A.pp[kp.pp_off + v];
+        const double s = P.p + Q.p;
+        const double rs = 1.0 / sqrt(s);
+        if constexpr (MODE != kSchwarz) {
+          if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;   // uniform
+        }
+        const double alpha = P.p * Q.p / s;
+        const double X = P.Px - Q.Px, Y = P.Py - Q.Py, Z = P.Pz - Q.Pz;
+        const double T = alpha * (X * X + Y * Y + Z * Z);
+        const double pref = kTwoPi25 * P.cc * Q.cc * rs;
+
+        // g_m = (-alpha)^m F_m(T) for m <= L, one lane each, via LDS scratch
+        double* g = Wt;  // W is rebuilt below, so its space is free here
+        if (lane <= L) {
+          double pw = 1.0;
+          for (int m = 0; m < lane; ++m) pw *= -alpha;
+          g[lane] = pw * boys_one(lane, T, A.boys);
+        }
+        if (lane == 60) herm_axis_to_lds<LA, LB>(Ea + 0 * W_::EA, P.Px - sA.x, P.Px - sB.x, 0.5 / P.p);
+        if (lane == 61) herm_axis_to_lds<LA, LB>(Ea + 1 * W_::EA, P.Py - sA.y, P.Py - sB.y, 0.5 / P.p);
+        if (lane == 62) herm_axis_to_lds<LA, LB>(Ea + 2 * W_::EA, P.Pz - sA.z, P.Pz - sB.z, 0.5 / P.p);
+        if (lane == 63) {
+          herm_axis_to_lds<LC, LD>(Ec + 0 * W_::EC, Q.Px - sC.x, Q.Px - sD.x, 0.5 / Q.p);
+          herm_axis_to_lds<LC, LD>(Ec + 1 * W_::EC, Q.Py - sC.y, Q.Py - sD.y, 0.5 / Q.p);
+          herm_axis_to_lds<LC, LD>(Ec + 2 * W_::EC, Q.Pz - sC.z, Q.Pz - sD.z, 0.5 / Q.p);
+        }
+        wave_sync();
+        // R_{TUV} = sum_{ijk} c(T,i)(2X)^{T-2i} c(U,j)(2Y)^{U-2j} c(V,k)(2Z)^{V-2k} g_{T+U+V-i-j-k}
+        for (int e = lane; e < W_::NH; e += 64) {
+          const int tT = kHerm.t[e], tU = kHerm.u[e], tV = kHerm.v[e];
+          double r = 0.0;
+          for (int i = 0; 2 * i <= tT; ++i) {
+            double px = kHermC[tT][i];
+            for (int q = 0; q < tT - 2 * i; ++q) px *= 2.0 * X;
+            for (int j = 0; 2 * j <= tU; ++j) {
+              double py = px * kHermC[tU][j];
+              for (int q = 0; q < tU - 2 * j; ++q) py *= 2.0 * Y;
+              for (int k = 0; 2 * k <= tV; ++k) {
+                double pz = py * kHermC[tV][k];
+                for (int q = 0; q < tV - 2 * k; ++q) pz *= 2.0 * Z;
+                r += pz * g[tT + tU + tV - i - j - k];
+              }
+            }
+          }
+          R[e] = r;
+        }
+        wave_sync();
+        // W[cd][tuv]
+        for (int e = lane; e < W_::NW; e += 64) {
+          const int cd = e / W_::NHAB, h = e - cd * W_::NHAB;
+          const Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
+          const int t0 = kHerm.t[h], u0 = kHerm.u[h], v0 = kHerm.v[h];
+          const double* ex = Ec + 0 * W_::EC + (c.x * (LD + 1) + d.x) * (LC + LD + 1);
+          const double* ey = Ec + 1 * W_::EC + (c.y * (LD + 1) + d.y) * (LC + LD + 1);
+          const double* ez = Ec + 2 * W_::EC + (c.z * (LD + 1) + d.z) * (LC + LD + 1);
+          double w = 0.0;
+          for (int T_ = 0; T_ <= c.x + d.x; ++T_)
+            for (int U_ = 0; U_ <= c.y + d.y; ++U_) {
+              double wz = 0.0;
+              for (int V_ = 0; V_ <= c.z + d.z; ++V_) {
+                const double sg = ((T_ + U_ + V_) & 1) ? -1.0 : 1.0;
+                wz += sg * ez[V_] * R[hidx(t0 + T_, u0 + U_, v0 + V_)];
+              }
+              w += ex[T_] * ey[U_] * wz;
+            }
+          Wt[e] = w;
+        }
+        wave_sync();
+        // (ab|cd) += pref sum_{tuv} E^{ab}_{tuv} W[cd][tuv]
+#pragma unroll
+        for (int k = 0; k < W_::NOUT; ++k) {
+          const int o = lane + 64 * k;
+          if (o < NINT) {
+            const int ab = o / NCD, cd = o - ab * NCD;
+            const Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
+            const double* ex = Ea + 0 * W_::EA + (a.x * (LB + 1) + b.x) * (LAB + 1);
+            const double* ey = Ea + 1 * W_::EA + (a.y * (LB + 1) + b.y) * (LAB + 1);
+            const double* ez = Ea + 2 * W_::EA + (a.z * (LB + 1) + b.z) * (LAB + 1);
+            const double* w = Wt + cd * W_::NHAB;
+            double val = 0.0;
+            for (int t_ = 0; t_ <= a.x + b.x; ++t_)
+              for (int u_ = 0; u_ <= a.y + b.y; ++u_) {
+                double vz = 0.0;
+                for (int v_ = 0; v_ <= a.z + b.z; ++v_) vz += ez[v_] * w[hidx(t_, u_, v_)];
+                val += ex[t_] * ey[u_] * vz;
+              }
+            acc[k] += pref * val;
+          }
+        }
+        wave_sync();
+      }
+    }
+
+    const int fa = sA.bf_off, fb = sB.bf_off, fc = sC.bf_off, fd = sD.bf_off;
+    if constexpr (MODE == kSchwarz) {
+#pragma unroll
+      for (int k = 0; k < W_::NOUT; ++k) {
+        const int o = lane + 64 * k;
+        if (o < NINT) {
+          const int ab = o / NCD, cd = o - ab * NCD;
+          if (ab == cd) {
+            const int ia = ab / NB, jb = ab % NB;
+            const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
+            const double v = sqrt(fabs(acc[k] * sc * sc));
+            A.sch_out[(fa + ia) * n + fb + jb] = v;
+            A.sch_out[(fb + jb) * n + fa + ia] = v;
+          }
+        }
+      }
+    } else {
+      double deg = 1.0;
+      if (bp.a == bp.b) deg *= 0.5;
+      if (kp.a == kp.b) deg *= 0.5;
+      if (A.same && ib == ik) deg *= 0.5;
+      double* EE = Wt;
+#pragma unroll
+      for (int k = 0; k < W_::NOUT; ++k) {
+        const int o = lane + 64 * k;
+        if (o < NINT) {
+          const int ab = o / NCD, cd = o - ab * NCD;
+          EE[o] = acc[k] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
+                  A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
+        }
+      }
+      wave_sync();
+      // Coulomb
+      double* Dst = lds + W_::oD;
+      for (int e = lane; e < NAB; e += 64) Dst[e] = A.DJ[(fa + e / NB) * n + fb + e % NB];
+      for (int e = lane; e < NCD; e += 64) Dst[NAB + e] = A.DJ[(fc + e / ND) * n + fd + e % ND];
+      wave_sync();
+      for (int e = lane; e < NAB + NCD; e += 64) {
+        double s = 0.0;
+        if (e < NAB) {
+          for (int cd = 0; cd < NCD; ++cd) s += EE[e * NCD + cd] * Dst[NAB + cd];
+          fatomic(&A.AJ[(fa + e / NB) * n + fb + e % NB], 2.0 * s);
+        } else {
+          const int cd = e - NAB;
+          for (int ab = 0; ab < NAB; ++ab) s += EE[ab * NCD + cd] * Dst[ab];
+          fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * s);
+        }
+      }
+      // Exchange, per spin
+      constexpr int nspin = (MODE == kUHF) ? 2 : 1;
+      for (int sp = 0; sp < nspin; ++sp) {
+        const double* D = sp ? A.DB : A.DA;
+        double* AK = sp ? A.AKB : A.AKA;
+        double* dbd = Dst + NAB + NCD;        // D[b,d]  NB x ND
+        double* dad = dbd + NB * ND;          // D[a,d]  NA x ND
+        double* dbc = dad + NA * ND;          // D[b,c]  NB x NC
+        double* dac = dbc + NB * NC;          // D[a,c]  NA x NC
+        wave_sync();
+        for (int e = lane; e < NB * ND; e += 64) dbd[e] = D[(fb + e / ND) * n + fd + e % ND];
+        for (int e = lane; e < NA * ND; e += 64) dad[e] = D[(fa + e / ND) * n + fd + e % ND];
+        for (int e = lane; e < NB * NC; e += 64) dbc[e] = D[(fb + e / NC) * n + fc + e % NC];
+        for (int e = lane; e < NA * NC; e += 64) dac[e] = D[(fa + e / NC) * n + fc + e % NC];
+        wave_sync();
+        constexpr int NK = NA * NC + NB * NC + NA * ND + NB * ND;
+        for (int e = lane; e < NK; e += 64) {
+          double s = 0.0;
+          if (e < NA * NC) {                       // K[a,c] += sum_bd v D[b,d]
+            const int a = e / NC, c = e % NC;
+            for (int b = 0; b < NB; ++b)
+              for (int d = 0; d < ND; ++d) s += EE[(a * NB + b) * NCD + c * ND + d] * dbd[b * ND + d];
+            fatomic(&AK[(fa + a) * n + fc + c], s);
+          } else if (e < NA * NC + NB * NC) {      // K[b,c] += sum_ad v D[a,d]
+            const int r = e - NA * NC, b = r / NC, c = r % NC;
+            for (int a = 0; a < NA; ++a)
+              for (int d = 0; d < ND; ++d) s += EE[(a * NB + b) * NCD + c * ND + d] * dad[a * ND + d];
+            fatomic(&AK[(fb + b) * n + fc + c], s);
+          } else if (e < NA * NC + NB * NC + NA * ND) {  // K[a,d] += sum_bc v D[b,c]
+            const int r = e - NA * NC - NB * NC, a = r / ND, d = r % ND;
+            for (int b = 0; b < NB; ++b)
+              for (int c = 0; c < NC; ++c) s += EE[(a * NB + b) * NCD + c * ND + d] * dbc[b * NC + c];
+            fatomic(&AK[(fa + a) * n + fd + d], s);
+          } else {                                 // K[b,d] += sum_ac v D[a,c]
+            const int r = e - NA * NC - NB * NC - NA * ND, b = r / ND, d = r % ND;
+            for (int a = 0; a < NA; ++a)
+              for (int c = 0; c < NC; ++c) s += EE[(a * NB + b) * NCD + c * ND + d] * dac[a * NC + c];
+            fatomic(&AK[(fb + b) * n + fd + d], s);
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+}  // namespace sqjk

@@ -1,0 +1,196 @@
+// sq_md.h - device-side building blocks of the two-electron kernels.
+//
+// Integrals are evaluated with the McMurchie-Davidson scheme: Hermite
+// expansion coefficients E^{ij}_t per Cartesian axis, the Hermite Coulomb
+// tensor R_{tuv} from the Boys function, and the contraction
+//   (ab|cd) = 2 pi^{5/2}/(p q sqrt(p+q)) K_ab K_cd
+//             sum_{tuv} E^{ab}_{tuv} sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V}.
+// Every loop bound is a template parameter, so the whole primitive-quartet
+// evaluation unrolls into straight-line code with all intermediates in
+// registers.  The Boys function F_m(T) is read from a table that each
+// workgroup keeps in LDS (Taylor expansion about the nearest grid point plus
+// downward recursion).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+
+namespace sqmd {
+
+// ---------------------------------------------------------------- basics
+template <int V> using ic = std::integral_constant<int, V>;
+
+template <typename F, int... I>
+__host__ __device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(ic<I>{}), ...);
+}
+// compile-time for: f(ic<0>), ..., f(ic<N-1>)
+template <int N, typename F>
+__host__ __device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+// number of (t,u,v) with t+u+v <= L
+__host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+// Cartesian component order used by the GAMESS reader of the reference
+// (src/basis.c:464-498): s; x y z; xx yy zz xy xz yz; f in that file's order.
+struct Cart { int x, y, z; };
+__host__ __device__ constexpr Cart cart(int l, int i) {
+  constexpr Cart s[1] = {{0, 0, 0}};
+  constexpr Cart p[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  constexpr Cart d[6] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+  constexpr Cart f[10] = {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0},
+                          {2, 0, 1}, {1, 0, 2}, {0, 2, 1}, {0, 1, 2}, {1, 1, 1}};
+  return l == 0 ? s[i] : l == 1 ? p[i] : l == 2 ? d[i] : f[i];
+}
+
+// compact index of (t,u,v), t+u+v <= L, grouped by total order
+__host__ __device__ constexpr int hidx(int t, int u, int v) {
+  const int n = t + u + v;
+  // entries of lower total order, then position inside order n
+  const int below = n * (n + 1) * (n + 2) / 6;
+  const int m = u + v;           // t = n - m
+  return below + m * (m + 1) / 2 + v;
+}
+
+constexpr double kTwoPi25 = 34.986836655249725;  // 2 pi^{5/2}
+constexpr double kPrimCutoff = 1.0e-15;            // bare prefactor cut
+
+// ---------------------------------------------------------------- Boys
+// Table: F_m(k*kBoysStep) for k in [0,kBoysNT), m in [0,kBoysNM).
+constexpr int kBoysNT = 401;
+constexpr double kBoysStep = 0.1;
+constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 40
+constexpr int kBoysNM = 28;
+constexpr int kBoysTaylor = 9;  // terms of the Taylor expansion
+
+// LDS copy keeps the rows m in [0, M) with stride M (M = L + kBoysTaylor).
+template <int L>
+__device__ __forceinline__ void boys_load_lds(double* lds, const double* __restrict__ tab) {
+  constexpr int M = L + kBoysTaylor;
+  for (int e = threadIdx.x + threadIdx.y * blockDim.x; e < kBoysNT * M;
+       e += blockDim.x * blockDim.y) {
+    const int k = e / M, m = e - k * M;
+    lds[e] = tab[k * kBoysNM + m];
+  }
+}
+
+// F_0..F_L at T.
+template <int L>
+__host__ __device__ __forceinline__ void boys(double T, double (&F)[L + 1], const double* lds) {
+  constexpr int M = L + kBoysTaylor;
+  if (T < kBoysTMax - 0.5 * kBoysStep) {
+    const int k = (int)(T * (1.0 / kBoysStep) + 0.5);
+    const double d = k * kBoysStep - T;  // expansion in (T_k - T)
+    const double* row = lds + k * M + L;
+    // F_L(T) = sum_j F_{L+j}(T_k) (T_k - T)^j / j!
+    double s = row[kBoysTaylor - 1];
+    sfor<kBoysTaylor - 1>([&](auto jj) {
+      constexpr int j = kBoysTaylor - 2 - decltype(jj)::value;
+      s = s * (d * (1.0 / (j + 1))) + row[j];
+    });
+    F[L] = s;
+    if constexpr (L > 0) {
+      const double e = exp(-T), t2 = T + T;
+      sfor<L>([&](auto mm) {
+        constexpr int m = L - 1 - decltype(mm)::value;
+        F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+      });
+    }
+  } else {
+    // large T: F_0 = sqrt(pi/T)/2, upward recursion (stable here)
+    const double rt = 1.0 / T;
+    F[0] = 0.88622692545275801365 * sqrt(rt);
+    if constexpr (L > 0) {
+      const double e = exp(-T), h = 0.5 * rt;
+      sfor<L>([&](auto mm) {
+        constexpr int m = decltype(mm)::value;
+        F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+      });
+    }
+  }
+}
+
+// ------------------------------------------------------ Hermite coefficients
+// E[i][j][t], i<=LI, j<=LJ, t<=i+j  (unit K_ab factored out)
+template <int LI, int LJ>
+struct Herm1 {
+  double e[LI + 1][LJ + 1][LI + LJ + 1];
+};
+
+template <int LI, int LJ>
+__host__ __device__ __forceinline__ void herm_coef(Herm1<LI, LJ>& E, double xpa, double xpb, double h2p) {
+  E.e[0][0][0] = 1.0;
+  sfor<LI>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    sfor<i + 2>([&](auto tt) {
+      constexpr int t = decltype(tt)::value;
+      double v = 0.0;
+      if constexpr (t >= 1) v += h2p * E.e[i][0][t - 1];
+      if constexpr (t <= i) v += xpa * E.e[i][0][t];
+      if constexpr (t + 1 <= i) v += (t + 1) * E.e[i][0][t + 1];
+      E.e[i + 1][0][t] = v;
+    });
+  });
+  sfor<LI + 1>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    sfor<LJ>([&](auto jj) {
+      constexpr int j = decltype(jj)::value;
+      sfor<i + j + 2>([&](auto tt) {
+        constexpr int t = decltype(tt)::value;
+        double v = 0.0;
+        if constexpr (t >= 1) v += h2p * E.e[i][j][t - 1];
+        if constexpr (t <= i + j) v += xpb * E.e[i][j][t];
+        if constexpr (t + 1 <= i + j) v += (t + 1) * E.e[i][j][t + 1];
+        E.e[i][j + 1][t] = v;
+      });
+    });
+  });
+}
+
+// ------------------------------------------------------ Hermite Coulomb R
+// R^{(0)}_{tuv} for t+u+v <= L, built level by level from
+// R^{(n)}_{000} = (-2 alpha)^n F_n.
+template <int L>
+__host__ __device__ __forceinline__ void herm_R(double (&R)[nherm(L)], const double (&F)[L + 1],
+                                       double alpha, double X, double Y, double Z) {
+  double lev[L + 1][nherm(L)];
+  double m2a = -2.0 * alpha, pw = 1.0;
+  sfor<L + 1>([&](auto nn) {
+    constexpr int n = decltype(nn)::value;
+    lev[n][0] = pw * F[n];
+    pw *= m2a;
+  });
+  // level n holds orders up to L-n; build from n = L-1 down to 0
+  sfor<L>([&](auto kk) {
+    constexpr int n = L - 1 - decltype(kk)::value;
+    constexpr int top = L - n;
+    sfor<top>([&](auto oo) {
+      constexpr int o = decltype(oo)::value + 1;  // total order 1..top
+      sfor<o + 1>([&](auto mm) {
+        constexpr int m = decltype(mm)::value;     // u+v
+        sfor<m + 1>([&](auto vv) {
+          constexpr int v = decltype(vv)::value;
+          constexpr int u = m - v, t = o - m;
+          double r;
+          if constexpr (t > 0) {
+            r = X * lev[n + 1][hidx(t - 1, u, v)];
+            if constexpr (t > 1) r += (t - 1) * lev[n + 1][hidx(t - 2, u, v)];
+          } else if constexpr (u > 0) {
+            r = Y * lev[n + 1][hidx(t, u - 1, v)];
+            if constexpr (u > 1) r += (u - 1) * lev[n + 1][hidx(t, u - 2, v)];
+          } else {
+            r = Z * lev[n + 1][hidx(t, u, v - 1)];
+            if constexpr (v > 1) r += (v - 1) * lev[n + 1][hidx(t, u, v - 2)];
+          }
+          lev[n][hidx(t, u, v)] = r;
+        });
+      });
+    });
+  });
+  sfor<nherm(L)>([&](auto ii) { R[decltype(ii)::value] = lev[0][decltype(ii)::value]; });
+}
+
+}  // namespace sqmd

@@ -1,0 +1,102 @@
+"""ctypes bindings of include/sq_jk.h (context API and drop-in entry points)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import GTOBasis, Option, dptr
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PD = C.POINTER(C.c_double)
+PG = C.POINTER(GTOBasis)
+_lib = None
+
+
+def library_path():
+    return os.path.join(PKG, "lib", "libsqamd.so")
+
+
+class SqjkStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("quartets", C.c_longlong),
+                ("quartets_eval", C.c_longlong), ("launches", C.c_int)]
+
+
+def lib():
+    """Load lib/libsqamd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    L.sqjk_create.argtypes = [C.c_int, PG, PD, C.c_int, C.POINTER(C.c_void_p)]
+    L.sqjk_build.argtypes = [C.c_void_p, PD, PD, C.c_double, C.c_double, C.c_int, PD, PD, PD]
+    L.sqjk_schwarz.argtypes = [C.c_void_p, PD]
+    L.sqjk_last_stats.argtypes = [C.c_void_p, C.POINTER(SqjkStats)]
+    L.sqjk_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.sqjk_destroy.argtypes = [C.c_void_p]
+    L.sqjk_errmsg.restype = C.c_char_p
+    L.GTO_JK_Matrix_Quartet_Parallel.argtypes = [C.c_int, C.c_int, PD, PD, PG, PD, C.c_double,
+                                                 PD, PD, PD, C.POINTER(Option)]
+    L.uhf_getGMatrix.argtypes = [C.c_int, PG, PD, C.c_double, PD, PD, PD, PD, PD, C.POINTER(Option)]
+    L.create_Schwarz.argtypes = [C.c_int, PG]
+    L.create_Schwarz.restype = PD
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {lib().sqjk_errmsg().decode()}")
+
+
+class JKEngine:
+    """One molecule's J/K context on the device (sqjk_create/sqjk_build)."""
+
+    def __init__(self, basis_ptr, n, schwarz=None, device=0):
+        L = lib()
+        self.n = n
+        self._h = C.c_void_p()
+        sch = None if schwarz is None else np.ascontiguousarray(schwarz, dtype=np.float64)
+        _check(L.sqjk_create(n, basis_ptr, dptr(sch) if sch is not None else None, device,
+                             C.byref(self._h)), "sqjk_create")
+
+    def build(self, PA, PB=None, fixed_cutoff=0.0, scf_cutoff=0.0, restricted=True, out=None):
+        """Returns (JT, KA, KB) accumulated onto `out` (zeros by default)."""
+        n = self.n
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = PA if PB is None else np.ascontiguousarray(PB, dtype=np.float64)
+        if out is None:
+            out = (np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n)))
+        J, KA, KB = out
+        _check(lib().sqjk_build(self._h, dptr(PA), dptr(PB), fixed_cutoff, scf_cutoff,
+                                int(bool(restricted)), dptr(J), dptr(KA), dptr(KB)), "sqjk_build")
+        return J, KA, KB
+
+    def schwarz(self):
+        s = np.zeros((self.n, self.n))
+        _check(lib().sqjk_schwarz(self._h, dptr(s)), "sqjk_schwarz")
+        return s
+
+    def stats(self):
+        st = SqjkStats()
+        _check(lib().sqjk_last_stats(self._h, C.byref(st)), "sqjk_last_stats")
+        return {k: getattr(st, k) for k, _ in SqjkStats._fields_}
+
+    def dims(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().sqjk_dims(self._h, C.byref(a), C.byref(b), C.byref(c)), "sqjk_dims")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if self._h:
+            lib().sqjk_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
