@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""J/K builds per second of the MI355X two-electron engine.
+
+Workload (one "step" = one full J/K build, inputs resident on the host as in
+the reference's uhf_getGMatrix call): seeded water cluster (H2O)32
+(seed 3201, rigid monomers, O-O >= 2.7 A, 1 g/cm^3), RHF/6-31G* with
+Cartesian d, 608 basis functions; Schwarz and density-weighted screening at
+1e-12 (the reference's rules); seeded random symmetric density; FP64.
+
+Prints one JSON line.  `roofline` is the FP64 vector-ALU roofline of the
+integral kernels (modelled flops per build / HIP-event kernel time), and
+`cpu_baseline` times the reference's own code (oracle/_ref, built from the
+reference sources) on a bounded slice of the same build on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "siam-quantum_amd"))
+
+METRIC = "J/K builds/s & SCF solves/s (p50/p99 lat), mixed vs FP64 GPU at |ΔE|≤1µEh"
+PEAK_FP64_TFLOPS = 78.6
+WORKLOADS = {
+    "w32": (32, 3201),
+    "w16": (16, 1601),
+}
+
+
+def make_inputs(name):
+    from sqamd import inputs
+    nmol, seed = WORKLOADS[name]
+    Z, X = inputs.water_cluster(nmol, seed)
+    mol = inputs.mol_from_angstrom(Z, X)
+    bas = inputs.make_basis(mol, "631g_star")
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((bas.n, bas.n)) * 0.01
+    return mol, bas, A + A.T
+
+
+def cpu_baseline(bas, P, cut, slices):
+    """Reference code (oracle/_ref) on the PQ % slices == 0 part of the same
+    build, one core; builds/s scaled by the number of slices."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference, default_option
+    from sqamd import dptr
+    ref = Reference()
+    n = bas.n
+    sch = ref.schwarz(bas.ptr, n)
+    opt = default_option(SCFCutoff=cut, restricted=1, maxMem=1, nCPU=slices)
+    ref.jk_reset(opt)
+    J, K1, K2 = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    t0 = time.perf_counter()
+    ref.L.GTO_JK_Matrix_Quartet_Parallel(0, n, dptr(P), dptr(P), bas.ptr, dptr(sch), cut,
+                                         dptr(J), dptr(K1), dptr(K2), C.byref(opt))
+    dt = time.perf_counter() - t0
+    ref.jk_reset(opt)
+    return {"value": 1.0 / (dt * slices), "unit": "builds/s", "cores": 1, "kind": "reference",
+            "sample": f"reference GTO_JK_Matrix_Quartet_Parallel, childID 0 of nCPU={slices} "
+                      f"(bra shell pairs PQ % {slices} == 0) of the same build, {dt:.1f} s, x{slices}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="w32", choices=sorted(WORKLOADS))
+    ap.add_argument("--cutoff", type=float, default=1e-12)
+    ap.add_argument("--cpu-slices", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from sqamd import JKEngine, workmodel
+    mol, bas, P = make_inputs(a.workload)
+    n = bas.n
+    eng = JKEngine(bas.ptr, n)
+    for _ in range(a.warmup):
+        eng.build(P, P, a.cutoff, a.cutoff, True)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.build(P, P, a.cutoff, a.cutoff, True)
+        kms.append(eng.stats()["kernel_ms"])
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    cls = eng.class_stats()
+    flops = workmodel.build_flops(cls)
+    kernel_s = float(np.mean(kms)) / 1e3
+    achieved = flops / kernel_s / 1e12
+    ms = wall / a.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": a.steps / wall,
+        "unit": "J/K builds/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"(H2O){WORKLOADS[a.workload][0]} RHF/6-31G* full J/K build, "
+                               f"{n} basis functions, seed {WORKLOADS[a.workload][1]}, "
+                               f"Schwarz+density screening {a.cutoff:g}, FP64",
+                   "basis_functions": n, "quartets": st["quartets"],
+                   "integrals": workmodel.integrals(cls)},
+        "kernel_ms": kernel_s * 1e3,
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_build": flops},
+    }
+    eng.close()
+    if not a.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(bas, P, a.cutoff, a.cpu_slices)
+        except FileNotFoundError as e:
+            out["cpu_baseline"] = {"value": None, "unit": "builds/s", "cores": 1, "kind": "reference",
+                                   "sample": f"not run: {e}"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

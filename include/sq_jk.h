@@ -69,6 +69,13 @@ int sqjk_schwarz(sqjk_ctx *ctx, double *out);
 /* Statistics of the last build. */
 int sqjk_last_stats(const sqjk_ctx *ctx, sqjk_stats *st);
 
+/* Work of the last build per angular class (21 classes for l <= 2):
+ * lq[4k..4k+3] = (la,lb,lc,ld), quartets[k] = contracted shell quartets
+ * launched (after Schwarz screening, before the density screen and the
+ * primitive cut-off), prim_quartets[k] = their primitive quartets.  Any
+ * output pointer may be NULL.  Returns the number of classes. */
+int sqjk_class_stats(const sqjk_ctx *ctx, int *lq, long long *quartets, long long *prim_quartets);
+
 /* Number of basis functions, reference shells and pure-l sub-shells. */
 int sqjk_dims(const sqjk_ctx *ctx, int *nBasis, int *nShell, int *nSubShell);
 

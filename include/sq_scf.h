@@ -1,0 +1,84 @@
+/*
+ * sq_scf.h - host-side drop-ins around the J/K engine: SCF driver, density
+ * DIIS, eigen-solvers, one-electron integrals, Boys function and input
+ * readers.  Each declaration keeps the reference signature it replaces.
+ * The J/K builds inside uhf()/rhf() and the eigen-solves run on the HIP
+ * device; the one-electron integrals and readers are host C, as in the
+ * reference.
+ */
+#ifndef SQ_SCF_H
+#define SQ_SCF_H
+
+#include <stdio.h>
+#include "sq_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- SCF (src/uhf.h:93-103, src/uhf.c:1171-1964) ---- */
+double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, int nEB,
+           double *CA, double *CB, double *eA, double *eB, struct option_t *opt);
+/* src/uhf.h:24 (src/uhf.c:77-90): P = sum_{k<nOcc} C_k C_k^T */
+void uhf_getDMatrix(int nBasis, int nOcc, double *C, double *P);
+
+/* ---- legacy RHF (src/rhf.h:26-47, src/rhf.c:59-72, 286-548) ---- */
+void rhf_getDMatrix(int nBasis, int nOcc, double *C, double *P);
+double rhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nE,
+           double *C, double *e, struct option_t *opt);
+
+/* ---- convergence (src/conv.h:34-37, src/conv.c:606-844) ----
+ * conv_diis4(0, 0.0, NULL, NULL) resets the history. */
+double conv_diis4(int nBasis, double alpha, double *PA, double *PB);
+
+/* ---- dense linear algebra (src/lin.h:20-33) ---- */
+int gen_sym_eigen(int nDim, const double *A, const double *S, double *e, double *C);
+int gen_sym_SomeEigen(int nDim, int nEigen, const double *A, const double *S, double *e, double *C);
+int linear_solver(int nRow, int nCol, const double *A, double *B);
+/* Device eigen-solver behind the two calls above; 0 on success. */
+int sqla_gen_eigen(int n, const double *F, const double *S, double *e, double *C);
+
+/* ---- Boys function (src/fgamma.h:19-24) ---- */
+double fgamma_0(double x);
+double fgamma(int m, double x);
+void fgamma_set(int max, double x, double *fset);
+
+/* ---- primitive one-electron integrals (src/int.h:20-48) ---- */
+double overlap(double alpha1, int l1, int m1, int n1, double xa, double ya, double za,
+               double alpha2, int l2, int m2, int n2, double xb, double yb, double zb);
+double moment(double alpha1, int l1, int m1, int n1, double xa, double ya, double za,
+              double alpha2, int l2, int m2, int n2, double xb, double yb, double zb,
+              int mx, int my, int mz, double xc, double yc, double zc);
+double kinetic(double alpha1, int l1, int m1, int n1, double xa, double ya, double za,
+               double alpha2, int l2, int m2, int n2, double xb, double yb, double zb);
+double nai(double x1, double y1, double z1, double norm1, int l1, int m1, int n1, double alpha1,
+           double x2, double y2, double z2, double norm2, int l2, int m2, int n2, double alpha2,
+           double x3, double y3, double z3);
+/* primitive ERI with norms (src/int.h:64-73) and without (src/int.h:75-79;
+ * the reference's primeERI omits the 2 pi^{5/2} K_ab K_cd/(pq) factor) */
+double eri(double xa, double ya, double za, double norma, int la, int ma, int na, double alphaa,
+           double xb, double yb, double zb, double normb, int lb, int mb, int nb, double alphab,
+           double xc, double yc, double zc, double normc, int lc, int mc, int nc, double alphac,
+           double xd, double yd, double zd, double normd, int ld, int md, int nd, double alphad);
+
+/* ---- contracted one-electron matrix elements (src/matrix.h:29-46) ---- */
+double GTO_overlap(int i, int j, const struct GTOBasis_t *gto);
+double GTO_moment(int i, int j, const struct GTOBasis_t *gto, int mx, int my, int mz,
+                  double xc, double yc, double zc);
+double GTO_kinetic(int i, int j, const struct GTOBasis_t *gto);
+double GTO_nuclei(int i, int j, const struct GTOBasis_t *gto, const struct Molecule_t *mol);
+
+/* ---- input (src/mol.h:31-37, src/basis.h:57-73) ---- */
+struct Molecule_t *readMolecule_XYZ(FILE *inFile);
+int get_nElectron(struct Molecule_t *mol);
+double nuclei_coulomb(const struct Molecule_t *mol);
+int get_nEA(struct Molecule_t *mol, int M);
+int get_nEB(struct Molecule_t *mol, int M);
+struct GTOBasisSet_t *read_GAMESS_BasisSet(FILE *inFile, char *name, int *nBasisSet);
+struct GTOBasis_t *genBasis(struct Molecule_t *mol, int *nBasis, int dbSize,
+                            const struct GTOBasisSet_t *basisDB);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SQ_SCF_H */

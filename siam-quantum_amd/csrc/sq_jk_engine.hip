@@ -139,6 +139,9 @@ struct sqjk_ctx {
   std::vector<long long> h_off;
   std::vector<double> h_DJ, h_PM, h_J, h_KA, h_KB;
   sqjk_stats last{};
+  // per class pair (X,Y) of the last build: contracted and primitive quartets
+  long long cls_q[kNPairClass][kNPairClass] = {};
+  long long cls_p[kNPairClass][kNPairClass] = {};
 };
 
 namespace {
@@ -409,12 +412,17 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   std::vector<Launch> plan;
   c->h_off.clear();
   long long nq = 0;
+  std::memset(c->cls_q, 0, sizeof(c->cls_q));
+  std::memset(c->cls_p, 0, sizeof(c->cls_p));
+  std::vector<long long> kpre;
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       const auto &bra = c->pairs[X], &ket = c->pairs[Y];
       if (bra.empty() || ket.empty()) continue;
       const size_t base = c->h_off.size();
-      long long acc = 0;
+      long long acc = 0, prim = 0;
+      kpre.assign(ket.size() + 1, 0);  // prefix sums of ket primitive pairs
+      for (size_t j = 0; j < ket.size(); ++j) kpre[j + 1] = kpre[j] + ket[j].npp;
       for (size_t i = 0; i < bra.size(); ++i) {
         c->h_off.push_back(acc);
         // ket list is sorted by sch descending: count j with sch_i*sch_j >= thr
@@ -426,8 +434,11 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
         size_t cnt = lo;
         if (X == Y) cnt = std::min(cnt, i + 1);
         acc += (long long)cnt;
+        prim += (long long)bra[i].npp * kpre[cnt];
       }
       c->h_off.push_back(acc);
+      c->cls_q[X][Y] = acc;
+      c->cls_p[X][Y] = prim;
       if (acc > 0) plan.push_back({X, Y, base, acc});
       nq += acc;
     }
@@ -503,6 +514,21 @@ int sqjk_last_stats(const sqjk_ctx* c, sqjk_stats* st) {
   if (!c || !st) return fail(SQJK_EARG, "sqjk_last_stats: null argument");
   *st = c->last;
   return SQJK_OK;
+}
+
+int sqjk_class_stats(const sqjk_ctx* c, int* lq, long long* quartets, long long* prim_quartets) {
+  if (!c) return fail(SQJK_EARG, "sqjk_class_stats: null context");
+  int k = 0;
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++k) {
+      if (lq) {
+        lq[4 * k + 0] = kPairL[X][0]; lq[4 * k + 1] = kPairL[X][1];
+        lq[4 * k + 2] = kPairL[Y][0]; lq[4 * k + 3] = kPairL[Y][1];
+      }
+      if (quartets) quartets[k] = c->cls_q[X][Y];
+      if (prim_quartets) prim_quartets[k] = c->cls_p[X][Y];
+    }
+  return k;
 }
 
 int sqjk_dims(const sqjk_ctx* c, int* nBasis, int* nShell, int* nSubShell) {

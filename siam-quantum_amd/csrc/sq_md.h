@@ -37,13 +37,17 @@ __host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L +
 // Cartesian component order used by the GAMESS reader of the reference
 // (src/basis.c:464-498): s; x y z; xx yy zz xy xz yz; f in that file's order.
 struct Cart { int x, y, z; };
+// Component i of angular momentum l, by arithmetic only (no lookup arrays,
+// so that the value is the same in every compilation context).
 __host__ __device__ constexpr Cart cart(int l, int i) {
-  constexpr Cart s[1] = {{0, 0, 0}};
-  constexpr Cart p[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  constexpr Cart d[6] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
-  constexpr Cart f[10] = {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0},
-                          {2, 0, 1}, {1, 0, 2}, {0, 2, 1}, {0, 1, 2}, {1, 1, 1}};
-  return l == 0 ? s[i] : l == 1 ? p[i] : l == 2 ? d[i] : f[i];
+  return l == 0 ? Cart{0, 0, 0}
+       : l == 1 ? Cart{i == 0 ? 1 : 0, i == 1 ? 1 : 0, i == 2 ? 1 : 0}
+       : l == 2 ? (i < 3 ? Cart{i == 0 ? 2 : 0, i == 1 ? 2 : 0, i == 2 ? 2 : 0}
+                         : Cart{i == 5 ? 0 : 1, i == 4 ? 0 : 1, i == 3 ? 0 : 1})
+       : (i == 0 ? Cart{3, 0, 0} : i == 1 ? Cart{0, 3, 0} : i == 2 ? Cart{0, 0, 3}
+        : i == 3 ? Cart{2, 1, 0} : i == 4 ? Cart{1, 2, 0} : i == 5 ? Cart{2, 0, 1}
+        : i == 6 ? Cart{1, 0, 2} : i == 7 ? Cart{0, 2, 1} : i == 8 ? Cart{0, 1, 2}
+        : Cart{1, 1, 1});
 }
 
 // compact index of (t,u,v), t+u+v <= L, grouped by total order

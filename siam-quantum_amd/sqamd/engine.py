@@ -37,6 +37,8 @@ def lib():
     L.sqjk_last_stats.argtypes = [C.c_void_p, C.POINTER(SqjkStats)]
     L.sqjk_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.sqjk_destroy.argtypes = [C.c_void_p]
+    L.sqjk_class_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_longlong)]
     L.sqjk_errmsg.restype = C.c_char_p
     L.GTO_JK_Matrix_Quartet_Parallel.argtypes = [C.c_int, C.c_int, PD, PD, PG, PD, C.c_double,
                                                  PD, PD, PD, C.POINTER(Option)]
@@ -84,6 +86,14 @@ class JKEngine:
         st = SqjkStats()
         _check(lib().sqjk_last_stats(self._h, C.byref(st)), "sqjk_last_stats")
         return {k: getattr(st, k) for k, _ in SqjkStats._fields_}
+
+    def class_stats(self):
+        """[(la,lb,lc,ld), quartets, primitive quartets] of the last build."""
+        lq = (C.c_int * 84)()
+        q = (C.c_longlong * 21)()
+        p = (C.c_longlong * 21)()
+        k = lib().sqjk_class_stats(self._h, lq, q, p)
+        return [((lq[4 * i], lq[4 * i + 1], lq[4 * i + 2], lq[4 * i + 3]), q[i], p[i]) for i in range(k)]
 
     def dims(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
