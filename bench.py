@@ -74,7 +74,20 @@ def main():
     ap.add_argument("--cutoff", type=float, default=1e-12)
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 times), to calibrate "
+                         "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
     a = ap.parse_args()
+    if a.copy_calibration_mib:
+        import torch
+        nbytes = a.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)

@@ -42,7 +42,7 @@ typedef struct {
 	double kernel_ms;      /* integral+digestion kernels, device time      */
 	double total_ms;       /* whole build incl. uploads/downloads (host)   */
 	long long quartets;    /* contracted shell quartets launched           */
-	long long quartets_eval; /* quartets surviving density screening (0 if not counted) */
+	long long quartets_fp32; /* of which evaluated in FP32 (mixed precision)  */
 	int launches;          /* kernel launches in this build                */
 } sqjk_stats;
 
@@ -62,6 +62,13 @@ int sqjk_create(int nBasis, const struct GTOBasis_t *gto, const double *schwarz_
  * primitive quartets with a bare prefactor below 1e-15 are dropped. */
 int sqjk_build(sqjk_ctx *ctx, const double *PA, const double *PB, double fixedCutoff,
                double scfCutoff, int restricted, double *JT, double *KA, double *KB);
+
+/* Mixed precision: shell quartets whose Schwarz product Sch_PQ*Sch_IJ is
+ * below `threshold` are evaluated in FP32 (Boys, Hermite and contraction);
+ * larger ones stay FP64, and all digestion and accumulation is FP64.
+ * threshold 0 (the default) keeps every quartet in FP64.  No counterpart in
+ * the reference (north-star mixed-precision mode). */
+int sqjk_set_mixed(sqjk_ctx *ctx, double threshold);
 
 /* Schwarz matrix sqrt|(pq|pq)| (n x n) as held by the context. */
 int sqjk_schwarz(sqjk_ctx *ctx, double *out);

@@ -17,15 +17,15 @@ using namespace sqjk;
 constexpr int LA = SQ_LA, LB = SQ_LB, LC = SQ_LC, LD = SQ_LD;
 using SI = SliceInfo<LA, LB, LC, LD>;
 
-template <int MODE, int S>
+template <int MODE, int S, typename R>
 int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (S >= SI::NSLICE || SI::NAB * SI::NCD > 81) {
     return 0;
   } else {
     constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
-    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S, R>), dim3(grid), dim3(256), lds, st, a);
     if (hipGetLastError() != hipSuccess) return -1;
-    const int rest = launch_from<MODE, S + 1>(a, grid, st);
+    const int rest = launch_from<MODE, S + 1, R>(a, grid, st);
     return rest < 0 ? -1 : rest + 1;
   }
 }
@@ -35,29 +35,31 @@ int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
 // one-wavefront-per-quartet kernel; the others the per-thread kernel.
 constexpr bool kWave = SI::NAB * SI::NCD > 81;
 
-template <int MODE>
+template <int MODE, typename R>
 int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_class_wave<LA, LB, LC, LD, MODE>), dim3(grid), dim3(64), 0, st, a);
+  hipLaunchKernelGGL((k_class_wave<LA, LB, LC, LD, MODE, R>), dim3(grid), dim3(64), 0, st, a);
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
-template <int MODE>
+template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
     // grid counts 256-thread blocks of the per-thread kernel; the wave
     // kernel uses one 64-thread workgroup per quartet in flight
-    return launch_wave<MODE>(a, grid * 4, st);
+    return launch_wave<MODE, R>(a, grid * 4, st);
   } else {
-    return launch_from<MODE, 0>(a, grid, st);
+    return launch_from<MODE, 0, R>(a, grid, st);
   }
 }
 
-int SQ_LAUNCHER(const sqjk::ClassArgs& a, int mode, int grid, hipStream_t st) {
+int SQ_LAUNCHER(const sqjk::ClassArgs& a, int mode, int fp32, int grid, hipStream_t st) {
   switch (mode) {
-    case sqjk::kRHF: return launch_mode<sqjk::kRHF>(a, grid, st);
-    case sqjk::kUHF: return launch_mode<sqjk::kUHF>(a, grid, st);
+    case sqjk::kRHF:
+      return fp32 ? launch_mode<sqjk::kRHF, float>(a, grid, st) : launch_mode<sqjk::kRHF, double>(a, grid, st);
+    case sqjk::kUHF:
+      return fp32 ? launch_mode<sqjk::kUHF, float>(a, grid, st) : launch_mode<sqjk::kUHF, double>(a, grid, st);
 #if SQ_LA == SQ_LC && SQ_LB == SQ_LD
-    case sqjk::kSchwarz: return launch_mode<sqjk::kSchwarz>(a, grid, st);
+    case sqjk::kSchwarz: return fp32 ? -1 : launch_mode<sqjk::kSchwarz, double>(a, grid, st);
 #endif
     default: return -1;
   }

@@ -13,12 +13,13 @@
 
 namespace sqjk {
 // Launches every slice of the class in `mode` (kRHF, kUHF or kSchwarz; the
-// latter only for LA==LC, LB==LD).  Returns the number of launches, or -1
-// if a launch failed.
-typedef int (*ClassLauncher)(const ClassArgs& a, int mode, int grid, hipStream_t st);
+// latter only for LA==LC, LB==LD), evaluating integrals in FP32 when fp32 is
+// set (kRHF/kUHF only).  Returns the number of launches, or -1 if a launch
+// failed.
+typedef int (*ClassLauncher)(const ClassArgs& a, int mode, int fp32, int grid, hipStream_t st);
 }
 
 #define SQJK_DECL(a, b, c, d) \
-  int sqjk_launch_##a##b##c##d(const sqjk::ClassArgs& args, int mode, int grid, hipStream_t st);
+  int sqjk_launch_##a##b##c##d(const sqjk::ClassArgs& args, int mode, int fp32, int grid, hipStream_t st);
 SQJK_CLASSES(SQJK_DECL)
 #undef SQJK_DECL

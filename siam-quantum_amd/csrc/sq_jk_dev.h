@@ -42,6 +42,7 @@ struct ClassArgs {
   const PairRec* __restrict__ ket;
   const PrimPair* __restrict__ pp;
   const long long* __restrict__ off;  // [nbra+1] quartet offsets per bra pair
+  const int* __restrict__ kstart;     // [nbra] first ket pair per bra pair (NULL: 0)
   const double* __restrict__ boys;    // Boys table (global)
   const double* __restrict__ DJ;      // density for J (PA+PB)
   const double* __restrict__ DA;      // alpha density for K
@@ -76,7 +77,9 @@ __host__ __device__ __forceinline__ void fatomic(double* p, double v) {
 #endif
 }
 
-template <int LA, int LB, int LC, int LD, int AB0, int AB1, int MODE>
+// R: precision of the integral evaluation (double, or float for the
+// mixed-precision range); digestion and accumulation are always FP64.
+template <int LA, int LB, int LC, int LD, int AB0, int AB1, int MODE, typename R = double>
 __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, const double* lds) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
@@ -86,16 +89,16 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
   const PairRec kp = A.ket[ik];
   const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
 
-  double acc[NS][NCD];
-  sfor<NS>([&](auto s) { sfor<NCD>([&](auto c) { acc[s][c] = 0.0; }); });
+  R acc[NS][NCD];
+  sfor<NS>([&](auto s) { sfor<NCD>([&](auto c) { acc[s][c] = R(0); }); });
 
   for (int u = 0; u < bp.npp; ++u) {
     const PrimPair P = A.pp[bp.pp_off + u];
-    const double ip = 1.0 / P.p;
-    Herm1<LA, LB> Ex, Ey, Ez;
-    herm_coef<LA, LB>(Ex, P.Px - sA.x, P.Px - sB.x, 0.5 * ip);
-    herm_coef<LA, LB>(Ey, P.Py - sA.y, P.Py - sB.y, 0.5 * ip);
-    herm_coef<LA, LB>(Ez, P.Pz - sA.z, P.Pz - sB.z, 0.5 * ip);
+    const R h2p = R(0.5 / P.p);
+    Herm1<LA, LB, R> Ex, Ey, Ez;
+    herm_coef<LA, LB, R>(Ex, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
+    herm_coef<LA, LB, R>(Ey, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
+    herm_coef<LA, LB, R>(Ez, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
     for (int v = 0; v < kp.npp; ++v) {
       const PrimPair Q = A.pp[kp.pp_off + v];
       const double s = P.p + Q.p;
@@ -103,42 +106,42 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       if constexpr (MODE != kSchwarz) {
         if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;
       }
-      const double alpha = P.p * Q.p / s;
-      const double X = P.Px - Q.Px, Y = P.Py - Q.Py, Z = P.Pz - Q.Pz;
-      const double T = alpha * (X * X + Y * Y + Z * Z);
-      double F[L + 1];
-      boys<L>(T, F, lds);
-      double R[nherm(L)];
-      herm_R<L>(R, F, alpha, X, Y, Z);
-      const double iq = 1.0 / Q.p;
-      Herm1<LC, LD> Fx, Fy, Fz;
-      herm_coef<LC, LD>(Fx, Q.Px - sC.x, Q.Px - sD.x, 0.5 * iq);
-      herm_coef<LC, LD>(Fy, Q.Py - sC.y, Q.Py - sD.y, 0.5 * iq);
-      herm_coef<LC, LD>(Fz, Q.Pz - sC.z, Q.Pz - sD.z, 0.5 * iq);
-      const double pref = kTwoPi25 * P.cc * Q.cc * rs;
+      const R alpha = R(P.p * Q.p / s);
+      const R X = R(P.Px - Q.Px), Y = R(P.Py - Q.Py), Z = R(P.Pz - Q.Pz);
+      const R T = alpha * (X * X + Y * Y + Z * Z);
+      R F[L + 1];
+      boys<L, R>(T, F, lds);
+      R Rt[nherm(L)];
+      herm_R<L, R>(Rt, F, alpha, X, Y, Z);
+      const R h2q = R(0.5 / Q.p);
+      Herm1<LC, LD, R> Fx, Fy, Fz;
+      herm_coef<LC, LD, R>(Fx, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
+      herm_coef<LC, LD, R>(Fy, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
+      herm_coef<LC, LD, R>(Fz, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
+      const R pref = R(kTwoPi25 * P.cc * Q.cc * rs);
 
       sfor<NCD>([&](auto cdI) {
         constexpr int cd = decltype(cdI)::value;
         constexpr Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
         constexpr int kx = c.x + d.x, ky = c.y + d.y, kz = c.z + d.z;
         // W[t][u][v] = sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V}
-        double W[LAB + 1][LAB + 1][LAB + 1];
+        R W[LAB + 1][LAB + 1][LAB + 1];
         sfor<LAB + 1>([&](auto tI) {
           constexpr int t = decltype(tI)::value;
           sfor<LAB + 1 - t>([&](auto uI) {
             constexpr int uu = decltype(uI)::value;
             sfor<LAB + 1 - t - uu>([&](auto vI) {
               constexpr int vv = decltype(vI)::value;
-              double w = 0.0;
+              R w = R(0);
               sfor<kx + 1>([&](auto TI) {
                 constexpr int T_ = decltype(TI)::value;
                 sfor<ky + 1>([&](auto UI) {
                   constexpr int U_ = decltype(UI)::value;
-                  double wz = 0.0;
+                  R wz = R(0);
                   sfor<kz + 1>([&](auto VI) {
                     constexpr int V_ = decltype(VI)::value;
-                    constexpr double sg = ((T_ + U_ + V_) & 1) ? -1.0 : 1.0;
-                    wz += sg * Fz.e[c.z][d.z][V_] * R[hidx(t + T_, uu + U_, vv + V_)];
+                    constexpr R sg = ((T_ + U_ + V_) & 1) ? R(-1) : R(1);
+                    wz += sg * Fz.e[c.z][d.z][V_] * Rt[hidx(t + T_, uu + U_, vv + V_)];
                   });
                   w += Fx.e[c.x][d.x][T_] * Fy.e[c.y][d.y][U_] * wz;
                 });
@@ -150,12 +153,12 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
         sfor<NS>([&](auto sI) {
           constexpr int ab = AB0 + decltype(sI)::value;
           constexpr Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
-          double val = 0.0;
+          R val = R(0);
           sfor<a.x + b.x + 1>([&](auto tI) {
             constexpr int t = decltype(tI)::value;
             sfor<a.y + b.y + 1>([&](auto uI) {
               constexpr int uu = decltype(uI)::value;
-              double vz = 0.0;
+              R vz = R(0);
               sfor<a.z + b.z + 1>([&](auto vI) {
                 constexpr int vv = decltype(vI)::value;
                 vz += Ez.e[a.z][b.z][vv] * W[t][uu][vv];
@@ -178,7 +181,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       constexpr int ab = AB0 + decltype(sI)::value;
       constexpr int ia = ab / NB, jb = ab % NB;
       const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
-      const double v = sqrt(fabs(acc[decltype(sI)::value][ab] * sc * sc));
+      const double v = sqrt(fabs((double)acc[decltype(sI)::value][ab] * sc * sc));
       A.sch_out[(fa + ia) * n + fb + jb] = v;
       A.sch_out[(fb + jb) * n + fa + ia] = v;
     });
@@ -193,13 +196,14 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     sfor<NC>([&](auto I) { sc_c[decltype(I)::value] = A.bscale[fc + decltype(I)::value]; });
     sfor<ND>([&](auto I) { sc_d[decltype(I)::value] = A.bscale[fd + decltype(I)::value]; });
 
-    // scaled integrals in place
+    // scaled integrals, FP64 from here on
+    double vv_[NS][NCD];
     sfor<NS>([&](auto sI) {
       constexpr int ab = AB0 + decltype(sI)::value;
       const double sab = deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB];
       sfor<NCD>([&](auto cI) {
         constexpr int cd = decltype(cI)::value;
-        acc[decltype(sI)::value][cd] *= sab * sc_c[cd / ND] * sc_d[cd % ND];
+        vv_[decltype(sI)::value][cd] = (double)acc[decltype(sI)::value][cd] * sab * sc_c[cd / ND] * sc_d[cd % ND];
       });
     });
 
@@ -218,8 +222,8 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       double jab = 0.0;
       sfor<NCD>([&](auto cI) {
         constexpr int cd = decltype(cI)::value;
-        jab += acc[s][cd] * dcd[cd];
-        jcd[cd] += acc[s][cd] * dab;
+        jab += vv_[s][cd] * dcd[cd];
+        jcd[cd] += vv_[s][cd] * dab;
       });
       fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * jab);
     });
@@ -253,7 +257,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
           sfor<ND>([&](auto dI) {
             constexpr int id = decltype(dI)::value;
             const int gd = fd + id;
-            const double v = acc[s][ic_ * ND + id];
+            const double v = vv_[s][ic_ * ND + id];
             kac[ia - A0][ic_] += v * D[gb * n + gd];
             kbc[jb - B0][ic_] += v * D[ga * n + gd];
             kad[ia - A0][id] += v * dbc;
@@ -289,7 +293,7 @@ __device__ __forceinline__ int find_bra(const long long* __restrict__ off, int n
 
 // One launch per (class, mode, slice S).  Grid-stride over the quartets of
 // the class; each workgroup first copies the Boys rows it needs into LDS.
-template <int LA, int LB, int LC, int LD, int MODE, int S>
+template <int LA, int LB, int LC, int LD, int MODE, int S, typename R = double>
 __global__ void __launch_bounds__(256) k_class(ClassArgs A) {
   using SI = SliceInfo<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(256) k_class(ClassArgs A) {
       ib = ik = (int)t;
     } else {
       ib = find_bra(A.off, A.nbra, t);
-      ik = (int)(t - A.off[ib]);
+      ik = (int)(t - A.off[ib]) + (A.kstart ? A.kstart[ib] : 0);
       // density-weighted screening at the reference-shell level
       const PairRec bp = A.bra[ib], kp = A.ket[ik];
       const double* PM = A.PM;
@@ -314,7 +318,7 @@ __global__ void __launch_bounds__(256) k_class(ClassArgs A) {
                         PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
       if (bp.sch * kp.sch * mp < A.fixed_cut) continue;
     }
-    quartet<LA, LB, LC, LD, AB0, AB1, MODE>(A, ib, ik, lds);
+    quartet<LA, LB, LC, LD, AB0, AB1, MODE, R>(A, ib, ik, lds);
   }
 }
 

@@ -135,8 +135,11 @@ struct sqjk_ctx {
   double *d_AJ = nullptr, *d_AKA = nullptr, *d_AKB = nullptr;
   double *d_J = nullptr, *d_KA = nullptr, *d_KB = nullptr, *d_sch = nullptr;
   long long* d_off = nullptr;
-  size_t off_cap = 0;
+  int* d_kstart = nullptr;
+  size_t off_cap = 0, ks_cap = 0;
   std::vector<long long> h_off;
+  std::vector<int> h_kstart;
+  double mixed_thr = 0.0;  // Schwarz products below this run in FP32
   std::vector<double> h_DJ, h_PM, h_J, h_KA, h_KB;
   sqjk_stats last{};
   // per class pair (X,Y) of the last build: contracted and primitive quartets
@@ -310,7 +313,7 @@ int compute_schwarz(sqjk_ctx* c) {
     a.nbra = (int)c->pairs[X].size();
     a.same = 1;
     ClassLauncher f = launcher_for(kPairL[X][0], kPairL[X][1], kPairL[X][0], kPairL[X][1]);
-    if (!f || f(a, kSchwarz, grid_for(a.total, 256), c->st) < 0)
+    if (!f || f(a, kSchwarz, 0, grid_for(a.total, 256), c->st) < 0)
       return fail(SQJK_EHIP, "Schwarz kernel launch failed");
   }
   c->sch_basis.resize(nn);
@@ -406,15 +409,19 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   if (uhf) HIPCHK(hipMemcpyAsync(c->d_DB, PB, nn * 8, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_PM, c->h_PM.data(), (size_t)nr * nr * 8, hipMemcpyHostToDevice, c->st));
 
-  // surviving quartet ranges per class pair
+  // surviving quartet ranges per class pair: ket pairs [0, c64) in FP64 and
+  // [c64, cnt) in FP32, where c64 counts Schwarz products >= mixed_thr
   const double thr = std::max(fixedCutoff, scfCutoff);
-  struct Launch { int X, Y; size_t off; long long total; };
+  const double thr64 = std::max(thr, c->mixed_thr);
+  struct Launch { int X, Y, fp32; size_t off, ks; long long total; };
   std::vector<Launch> plan;
   c->h_off.clear();
-  long long nq = 0;
+  c->h_kstart.clear();
+  long long nq = 0, nq32 = 0;
   std::memset(c->cls_q, 0, sizeof(c->cls_q));
   std::memset(c->cls_p, 0, sizeof(c->cls_p));
   std::vector<long long> kpre;
+  std::vector<size_t> c64v;
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       const auto &bra = c->pairs[X], &ket = c->pairs[Y];
@@ -423,24 +430,40 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
       long long acc = 0, prim = 0;
       kpre.assign(ket.size() + 1, 0);  // prefix sums of ket primitive pairs
       for (size_t j = 0; j < ket.size(); ++j) kpre[j + 1] = kpre[j] + ket[j].npp;
+      c64v.assign(bra.size(), 0);
+      // both lists are sorted by sch descending, so the number of ket pairs
+      // with sch_i*sch_j >= thr can only shrink as i grows: one sweep
+      size_t lo = ket.size(), lo64 = ket.size();
       for (size_t i = 0; i < bra.size(); ++i) {
         c->h_off.push_back(acc);
-        // ket list is sorted by sch descending: count j with sch_i*sch_j >= thr
-        size_t lo = 0, hi = ket.size();
-        while (lo < hi) {
-          const size_t mid = (lo + hi) / 2;
-          if (bra[i].sch * ket[mid].sch >= thr) lo = mid + 1; else hi = mid;
-        }
-        size_t cnt = lo;
-        if (X == Y) cnt = std::min(cnt, i + 1);
-        acc += (long long)cnt;
+        while (lo > 0 && !(bra[i].sch * ket[lo - 1].sch >= thr)) --lo;
+        while (lo64 > 0 && !(bra[i].sch * ket[lo64 - 1].sch >= thr64)) --lo64;
+        size_t cnt = lo, c64 = lo64;
+        if (X == Y) { cnt = std::min(cnt, i + 1); c64 = std::min(c64, i + 1); }
+        c64v[i] = c64;
+        acc += (long long)c64;
         prim += (long long)bra[i].npp * kpre[cnt];
       }
       c->h_off.push_back(acc);
-      c->cls_q[X][Y] = acc;
+      if (acc > 0) plan.push_back({X, Y, 0, base, 0, acc});
+      // FP32 range
+      const size_t base32 = c->h_off.size(), ks = c->h_kstart.size();
+      long long acc32 = 0;
+      lo = ket.size();
+      for (size_t i = 0; i < bra.size(); ++i) {
+        c->h_off.push_back(acc32);
+        c->h_kstart.push_back((int)c64v[i]);
+        while (lo > 0 && !(bra[i].sch * ket[lo - 1].sch >= thr)) --lo;
+        size_t cnt = lo;
+        if (X == Y) cnt = std::min(cnt, i + 1);
+        acc32 += (long long)(cnt > c64v[i] ? cnt - c64v[i] : 0);
+      }
+      c->h_off.push_back(acc32);
+      if (acc32 > 0) plan.push_back({X, Y, 1, base32, ks, acc32});
+      c->cls_q[X][Y] = acc + acc32;
       c->cls_p[X][Y] = prim;
-      if (acc > 0) plan.push_back({X, Y, base, acc});
-      nq += acc;
+      nq += acc + acc32;
+      nq32 += acc32;
     }
   if (c->off_cap < c->h_off.size()) {
     if (c->d_off) (void)hipFree(c->d_off);
@@ -448,8 +471,17 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     int rc = dalloc(&c->d_off, c->off_cap);
     if (rc) return rc;
   }
+  if (c->ks_cap < c->h_kstart.size() || !c->d_kstart) {
+    if (c->d_kstart) (void)hipFree(c->d_kstart);
+    c->ks_cap = std::max<size_t>(c->h_kstart.size(), 1024) * 2;
+    int rc = dalloc(&c->d_kstart, c->ks_cap);
+    if (rc) return rc;
+  }
   if (!c->h_off.empty())
     HIPCHK(hipMemcpyAsync(c->d_off, c->h_off.data(), c->h_off.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, c->st));
+  if (!c->h_kstart.empty())
+    HIPCHK(hipMemcpyAsync(c->d_kstart, c->h_kstart.data(), c->h_kstart.size() * sizeof(int),
                           hipMemcpyHostToDevice, c->st));
   const int fill_grid = grid_for((long long)nn, 256);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, nn, 0.0);
@@ -463,12 +495,13 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
     a.off = c->d_off + L.off;
+    a.kstart = L.fp32 ? c->d_kstart + L.ks : nullptr;
     a.nbra = (int)c->pairs[L.X].size();
     a.same = (L.X == L.Y);
     a.total = L.total;
     a.fixed_cut = fixedCutoff;
     ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
-    const int k = f ? f(a, uhf ? kUHF : kRHF, grid_for(L.total, 256), c->st) : -1;
+    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, grid_for(L.total, 256), c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
   }
@@ -499,8 +532,14 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   c->last.kernel_ms = ms;
   c->last.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->last.quartets = nq;
-  c->last.quartets_eval = 0;
+  c->last.quartets_fp32 = nq32;
   c->last.launches = launches;
+  return SQJK_OK;
+}
+
+int sqjk_set_mixed(sqjk_ctx* c, double threshold) {
+  if (!c || threshold < 0.0) return fail(SQJK_EARG, "sqjk_set_mixed: invalid arguments");
+  c->mixed_thr = threshold;
   return SQJK_OK;
 }
 
@@ -544,7 +583,7 @@ void sqjk_destroy(sqjk_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   void* ptrs[] = {c->d_sub, c->d_pp, c->d_boys, c->d_bscale, c->d_DJ, c->d_DA, c->d_DB, c->d_PM, c->d_AJ,
-                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_off};
+                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_off, c->d_kstart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int X = 0; X < kNPairClass; ++X)

@@ -55,15 +55,15 @@ __device__ __forceinline__ double boys_one(int n, double T, const double* __rest
   return f;
 }
 
-template <int LI, int LJ>
-__device__ __forceinline__ void herm_axis_to_lds(double* out, double xpa, double xpb, double h2p) {
-  Herm1<LI, LJ> E;
-  herm_coef<LI, LJ>(E, xpa, xpb, h2p);
+template <int LI, int LJ, typename R>
+__device__ __forceinline__ void herm_axis_to_lds(R* out, R xpa, R xpb, R h2p) {
+  Herm1<LI, LJ, R> E;
+  herm_coef<LI, LJ, R>(E, xpa, xpb, h2p);
   sfor<LI + 1>([&](auto i) {
     sfor<LJ + 1>([&](auto j) {
       sfor<LI + LJ + 1>([&](auto t) {
         out[(i.value * (LJ + 1) + j.value) * (LI + LJ + 1) + t.value] =
-            (t.value <= i.value + j.value) ? E.e[i.value][j.value][t.value] : 0.0;
+            (t.value <= i.value + j.value) ? E.e[i.value][j.value][t.value] : R(0);
       });
     });
   });
@@ -87,16 +87,18 @@ struct WaveLayout {
   static constexpr int NOUT = (NINT + 63) / 64;         // integrals per lane
 };
 
-template <int LA, int LB, int LC, int LD, int MODE>
+// R: precision of the integral evaluation; staging of the contracted block
+// and the digestion are FP64.
+template <int LA, int LB, int LC, int LD, int MODE, typename R = double>
 __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
   using W_ = WaveLayout<LA, LB, LC, LD>;
   constexpr int NA = W_::NA, NB = W_::NB, NC = W_::NC, ND = W_::ND;
   constexpr int NAB = W_::NAB, NCD = W_::NCD, NINT = W_::NINT, L = W_::L, LAB = W_::LAB;
   __shared__ double lds[W_::TOTAL];
-  double* R = lds + W_::oR;
-  double* Ea = lds + W_::oEa;
-  double* Ec = lds + W_::oEc;
-  double* Wt = lds + W_::oW;
+  R* Rh = reinterpret_cast<R*>(lds + W_::oR);    // views sized in doubles,
+  R* Ea = reinterpret_cast<R*>(lds + W_::oEa);   // so an FP32 view fits
+  R* Ec = reinterpret_cast<R*>(lds + W_::oEc);
+  R* Wt = reinterpret_cast<R*>(lds + W_::oW);
   const int lane = threadIdx.x;
   const int n = A.n;
 
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       ib = ik = (int)t;
     } else {
       ib = find_bra(A.off, A.nbra, t);
-      ik = (int)(t - A.off[ib]);
+      ik = (int)(t - A.off[ib]) + (A.kstart ? A.kstart[ib] : 0);
       const PairRec bp = A.bra[ib], kp = A.ket[ik];
       const double* PM = A.PM;
       const int nr = A.nref;
@@ -117,9 +119,9 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
     const PairRec bp = A.bra[ib], kp = A.ket[ik];
     const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
 
-    double acc[W_::NOUT];
+    R acc[W_::NOUT];
 #pragma unroll
-    for (int k = 0; k < W_::NOUT; ++k) acc[k] = 0.0;
+    for (int k = 0; k < W_::NOUT; ++k) acc[k] = R(0);
 
     for (int u = 0; u < bp.npp; ++u) {
       const PrimPair P = A.pp[bp.pp_off + u];
@@ -130,45 +132,47 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
         if constexpr (MODE != kSchwarz) {
           if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;   // uniform
         }
-        const double alpha = P.p * Q.p / s;
-        const double X = P.Px - Q.Px, Y = P.Py - Q.Py, Z = P.Pz - Q.Pz;
-        const double T = alpha * (X * X + Y * Y + Z * Z);
-        const double pref = kTwoPi25 * P.cc * Q.cc * rs;
+        const double alpha_d = P.p * Q.p / s;
+        const double Xd = P.Px - Q.Px, Yd = P.Py - Q.Py, Zd = P.Pz - Q.Pz;
+        const double T = alpha_d * (Xd * Xd + Yd * Yd + Zd * Zd);
+        const R X = R(Xd), Y = R(Yd), Z = R(Zd);
+        const R pref = R(kTwoPi25 * P.cc * Q.cc * rs);
 
         // g_m = (-alpha)^m F_m(T) for m <= L, one lane each, via LDS scratch
-        double* g = Wt;  // W is rebuilt below, so its space is free here
+        R* g = Wt;  // W is rebuilt below, so its space is free here
         if (lane <= L) {
           double pw = 1.0;
-          for (int m = 0; m < lane; ++m) pw *= -alpha;
-          g[lane] = pw * boys_one(lane, T, A.boys);
+          for (int m = 0; m < lane; ++m) pw *= -alpha_d;
+          g[lane] = R(pw * boys_one(lane, T, A.boys));
         }
-        if (lane == 60) herm_axis_to_lds<LA, LB>(Ea + 0 * W_::EA, P.Px - sA.x, P.Px - sB.x, 0.5 / P.p);
-        if (lane == 61) herm_axis_to_lds<LA, LB>(Ea + 1 * W_::EA, P.Py - sA.y, P.Py - sB.y, 0.5 / P.p);
-        if (lane == 62) herm_axis_to_lds<LA, LB>(Ea + 2 * W_::EA, P.Pz - sA.z, P.Pz - sB.z, 0.5 / P.p);
+        const R h2p = R(0.5 / P.p), h2q = R(0.5 / Q.p);
+        if (lane == 60) herm_axis_to_lds<LA, LB, R>(Ea + 0 * W_::EA, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
+        if (lane == 61) herm_axis_to_lds<LA, LB, R>(Ea + 1 * W_::EA, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
+        if (lane == 62) herm_axis_to_lds<LA, LB, R>(Ea + 2 * W_::EA, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
         if (lane == 63) {
-          herm_axis_to_lds<LC, LD>(Ec + 0 * W_::EC, Q.Px - sC.x, Q.Px - sD.x, 0.5 / Q.p);
-          herm_axis_to_lds<LC, LD>(Ec + 1 * W_::EC, Q.Py - sC.y, Q.Py - sD.y, 0.5 / Q.p);
-          herm_axis_to_lds<LC, LD>(Ec + 2 * W_::EC, Q.Pz - sC.z, Q.Pz - sD.z, 0.5 / Q.p);
+          herm_axis_to_lds<LC, LD, R>(Ec + 0 * W_::EC, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
+          herm_axis_to_lds<LC, LD, R>(Ec + 1 * W_::EC, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
+          herm_axis_to_lds<LC, LD, R>(Ec + 2 * W_::EC, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
         }
         wave_sync();
         // R_{TUV} = sum_{ijk} c(T,i)(2X)^{T-2i} c(U,j)(2Y)^{U-2j} c(V,k)(2Z)^{V-2k} g_{T+U+V-i-j-k}
         for (int e = lane; e < W_::NH; e += 64) {
           const int tT = kHerm.t[e], tU = kHerm.u[e], tV = kHerm.v[e];
-          double r = 0.0;
+          R r = R(0);
           for (int i = 0; 2 * i <= tT; ++i) {
-            double px = kHermC[tT][i];
-            for (int q = 0; q < tT - 2 * i; ++q) px *= 2.0 * X;
+            R px = R(kHermC[tT][i]);
+            for (int q = 0; q < tT - 2 * i; ++q) px *= R(2) * X;
             for (int j = 0; 2 * j <= tU; ++j) {
-              double py = px * kHermC[tU][j];
-              for (int q = 0; q < tU - 2 * j; ++q) py *= 2.0 * Y;
+              R py = px * R(kHermC[tU][j]);
+              for (int q = 0; q < tU - 2 * j; ++q) py *= R(2) * Y;
               for (int k = 0; 2 * k <= tV; ++k) {
-                double pz = py * kHermC[tV][k];
-                for (int q = 0; q < tV - 2 * k; ++q) pz *= 2.0 * Z;
+                R pz = py * R(kHermC[tV][k]);
+                for (int q = 0; q < tV - 2 * k; ++q) pz *= R(2) * Z;
                 r += pz * g[tT + tU + tV - i - j - k];
               }
             }
           }
-          R[e] = r;
+          Rh[e] = r;
         }
         wave_sync();
         // W[cd][tuv]
@@ -176,16 +180,16 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
           const int cd = e / W_::NHAB, h = e - cd * W_::NHAB;
           const Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
           const int t0 = kHerm.t[h], u0 = kHerm.u[h], v0 = kHerm.v[h];
-          const double* ex = Ec + 0 * W_::EC + (c.x * (LD + 1) + d.x) * (LC + LD + 1);
-          const double* ey = Ec + 1 * W_::EC + (c.y * (LD + 1) + d.y) * (LC + LD + 1);
-          const double* ez = Ec + 2 * W_::EC + (c.z * (LD + 1) + d.z) * (LC + LD + 1);
-          double w = 0.0;
+          const R* ex = Ec + 0 * W_::EC + (c.x * (LD + 1) + d.x) * (LC + LD + 1);
+          const R* ey = Ec + 1 * W_::EC + (c.y * (LD + 1) + d.y) * (LC + LD + 1);
+          const R* ez = Ec + 2 * W_::EC + (c.z * (LD + 1) + d.z) * (LC + LD + 1);
+          R w = R(0);
           for (int T_ = 0; T_ <= c.x + d.x; ++T_)
             for (int U_ = 0; U_ <= c.y + d.y; ++U_) {
-              double wz = 0.0;
+              R wz = R(0);
               for (int V_ = 0; V_ <= c.z + d.z; ++V_) {
-                const double sg = ((T_ + U_ + V_) & 1) ? -1.0 : 1.0;
-                wz += sg * ez[V_] * R[hidx(t0 + T_, u0 + U_, v0 + V_)];
+                const R sg = ((T_ + U_ + V_) & 1) ? R(-1) : R(1);
+                wz += sg * ez[V_] * Rh[hidx(t0 + T_, u0 + U_, v0 + V_)];
               }
               w += ex[T_] * ey[U_] * wz;
             }
@@ -199,14 +203,14 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
           if (o < NINT) {
             const int ab = o / NCD, cd = o - ab * NCD;
             const Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
-            const double* ex = Ea + 0 * W_::EA + (a.x * (LB + 1) + b.x) * (LAB + 1);
-            const double* ey = Ea + 1 * W_::EA + (a.y * (LB + 1) + b.y) * (LAB + 1);
-            const double* ez = Ea + 2 * W_::EA + (a.z * (LB + 1) + b.z) * (LAB + 1);
-            const double* w = Wt + cd * W_::NHAB;
-            double val = 0.0;
+            const R* ex = Ea + 0 * W_::EA + (a.x * (LB + 1) + b.x) * (LAB + 1);
+            const R* ey = Ea + 1 * W_::EA + (a.y * (LB + 1) + b.y) * (LAB + 1);
+            const R* ez = Ea + 2 * W_::EA + (a.z * (LB + 1) + b.z) * (LAB + 1);
+            const R* w = Wt + cd * W_::NHAB;
+            R val = R(0);
             for (int t_ = 0; t_ <= a.x + b.x; ++t_)
               for (int u_ = 0; u_ <= a.y + b.y; ++u_) {
-                double vz = 0.0;
+                R vz = R(0);
                 for (int v_ = 0; v_ <= a.z + b.z; ++v_) vz += ez[v_] * w[hidx(t_, u_, v_)];
                 val += ex[t_] * ey[u_] * vz;
               }
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
           if (ab == cd) {
             const int ia = ab / NB, jb = ab % NB;
             const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
-            const double v = sqrt(fabs(acc[k] * sc * sc));
+            const double v = sqrt(fabs((double)acc[k] * sc * sc));
             A.sch_out[(fa + ia) * n + fb + jb] = v;
             A.sch_out[(fb + jb) * n + fa + ia] = v;
           }
@@ -238,13 +242,13 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       if (bp.a == bp.b) deg *= 0.5;
       if (kp.a == kp.b) deg *= 0.5;
       if (A.same && ib == ik) deg *= 0.5;
-      double* EE = Wt;
+      double* EE = lds + W_::oW;
 #pragma unroll
       for (int k = 0; k < W_::NOUT; ++k) {
         const int o = lane + 64 * k;
         if (o < NINT) {
           const int ab = o / NCD, cd = o - ab * NCD;
-          EE[o] = acc[k] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
+          EE[o] = (double)acc[k] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
                   A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
         }
       }

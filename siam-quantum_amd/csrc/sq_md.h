@@ -81,37 +81,44 @@ __device__ __forceinline__ void boys_load_lds(double* lds, const double* __restr
   }
 }
 
-// F_0..F_L at T.
-template <int L>
-__host__ __device__ __forceinline__ void boys(double T, double (&F)[L + 1], const double* lds) {
+__host__ __device__ __forceinline__ double rexp(double x) { return exp(x); }
+__host__ __device__ __forceinline__ float rexp(float x) { return expf(x); }
+__host__ __device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ float rsqrt_(float x) { return sqrtf(x); }
+
+// F_0..F_L at T, evaluated in precision R (the table itself is FP64).
+// FP32 evaluation needs fewer Taylor terms for the same relative accuracy.
+template <int L, typename R = double>
+__host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* lds) {
   constexpr int M = L + kBoysTaylor;
-  if (T < kBoysTMax - 0.5 * kBoysStep) {
-    const int k = (int)(T * (1.0 / kBoysStep) + 0.5);
-    const double d = k * kBoysStep - T;  // expansion in (T_k - T)
+  constexpr int NT = sizeof(R) == 4 ? 5 : kBoysTaylor;  // 0.05^5/5! ~ 2.6e-9
+  if (T < R(kBoysTMax - 0.5 * kBoysStep)) {
+    const int k = (int)(T * R(1.0 / kBoysStep) + R(0.5));
+    const R d = R(k * kBoysStep) - T;  // expansion in (T_k - T)
     const double* row = lds + k * M + L;
     // F_L(T) = sum_j F_{L+j}(T_k) (T_k - T)^j / j!
-    double s = row[kBoysTaylor - 1];
-    sfor<kBoysTaylor - 1>([&](auto jj) {
-      constexpr int j = kBoysTaylor - 2 - decltype(jj)::value;
-      s = s * (d * (1.0 / (j + 1))) + row[j];
+    R s = R(row[NT - 1]);
+    sfor<NT - 1>([&](auto jj) {
+      constexpr int j = NT - 2 - decltype(jj)::value;
+      s = s * (d * R(1.0 / (j + 1))) + R(row[j]);
     });
     F[L] = s;
     if constexpr (L > 0) {
-      const double e = exp(-T), t2 = T + T;
+      const R e = rexp(-T), t2 = T + T;
       sfor<L>([&](auto mm) {
         constexpr int m = L - 1 - decltype(mm)::value;
-        F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
       });
     }
   } else {
     // large T: F_0 = sqrt(pi/T)/2, upward recursion (stable here)
-    const double rt = 1.0 / T;
-    F[0] = 0.88622692545275801365 * sqrt(rt);
+    const R rt = R(1) / T;
+    F[0] = R(0.88622692545275801365) * rsqrt_(rt);
     if constexpr (L > 0) {
-      const double e = exp(-T), h = 0.5 * rt;
+      const R e = rexp(-T), h = R(0.5) * rt;
       sfor<L>([&](auto mm) {
         constexpr int m = decltype(mm)::value;
-        F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+        F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;
       });
     }
   }
@@ -119,22 +126,22 @@ __host__ __device__ __forceinline__ void boys(double T, double (&F)[L + 1], cons
 
 // ------------------------------------------------------ Hermite coefficients
 // E[i][j][t], i<=LI, j<=LJ, t<=i+j  (unit K_ab factored out)
-template <int LI, int LJ>
+template <int LI, int LJ, typename R = double>
 struct Herm1 {
-  double e[LI + 1][LJ + 1][LI + LJ + 1];
+  R e[LI + 1][LJ + 1][LI + LJ + 1];
 };
 
-template <int LI, int LJ>
-__host__ __device__ __forceinline__ void herm_coef(Herm1<LI, LJ>& E, double xpa, double xpb, double h2p) {
-  E.e[0][0][0] = 1.0;
+template <int LI, int LJ, typename R>
+__host__ __device__ __forceinline__ void herm_coef(Herm1<LI, LJ, R>& E, R xpa, R xpb, R h2p) {
+  E.e[0][0][0] = R(1);
   sfor<LI>([&](auto ii) {
     constexpr int i = decltype(ii)::value;
     sfor<i + 2>([&](auto tt) {
       constexpr int t = decltype(tt)::value;
-      double v = 0.0;
+      R v = R(0);
       if constexpr (t >= 1) v += h2p * E.e[i][0][t - 1];
       if constexpr (t <= i) v += xpa * E.e[i][0][t];
-      if constexpr (t + 1 <= i) v += (t + 1) * E.e[i][0][t + 1];
+      if constexpr (t + 1 <= i) v += R(t + 1) * E.e[i][0][t + 1];
       E.e[i + 1][0][t] = v;
     });
   });
@@ -144,10 +151,10 @@ __host__ __device__ __forceinline__ void herm_coef(Herm1<LI, LJ>& E, double xpa,
       constexpr int j = decltype(jj)::value;
       sfor<i + j + 2>([&](auto tt) {
         constexpr int t = decltype(tt)::value;
-        double v = 0.0;
+        R v = R(0);
         if constexpr (t >= 1) v += h2p * E.e[i][j][t - 1];
         if constexpr (t <= i + j) v += xpb * E.e[i][j][t];
-        if constexpr (t + 1 <= i + j) v += (t + 1) * E.e[i][j][t + 1];
+        if constexpr (t + 1 <= i + j) v += R(t + 1) * E.e[i][j][t + 1];
         E.e[i][j + 1][t] = v;
       });
     });
@@ -157,11 +164,11 @@ __host__ __device__ __forceinline__ void herm_coef(Herm1<LI, LJ>& E, double xpa,
 // ------------------------------------------------------ Hermite Coulomb R
 // R^{(0)}_{tuv} for t+u+v <= L, built level by level from
 // R^{(n)}_{000} = (-2 alpha)^n F_n.
-template <int L>
-__host__ __device__ __forceinline__ void herm_R(double (&R)[nherm(L)], const double (&F)[L + 1],
-                                       double alpha, double X, double Y, double Z) {
-  double lev[L + 1][nherm(L)];
-  double m2a = -2.0 * alpha, pw = 1.0;
+template <int L, typename Rl = double>
+__host__ __device__ __forceinline__ void herm_R(Rl (&R)[nherm(L)], const Rl (&F)[L + 1],
+                                       Rl alpha, Rl X, Rl Y, Rl Z) {
+  Rl lev[L + 1][nherm(L)];
+  Rl m2a = Rl(-2) * alpha, pw = Rl(1);
   sfor<L + 1>([&](auto nn) {
     constexpr int n = decltype(nn)::value;
     lev[n][0] = pw * F[n];
@@ -178,16 +185,16 @@ __host__ __device__ __forceinline__ void herm_R(double (&R)[nherm(L)], const dou
         sfor<m + 1>([&](auto vv) {
           constexpr int v = decltype(vv)::value;
           constexpr int u = m - v, t = o - m;
-          double r;
+          Rl r;
           if constexpr (t > 0) {
             r = X * lev[n + 1][hidx(t - 1, u, v)];
-            if constexpr (t > 1) r += (t - 1) * lev[n + 1][hidx(t - 2, u, v)];
+            if constexpr (t > 1) r += Rl(t - 1) * lev[n + 1][hidx(t - 2, u, v)];
           } else if constexpr (u > 0) {
             r = Y * lev[n + 1][hidx(t, u - 1, v)];
-            if constexpr (u > 1) r += (u - 1) * lev[n + 1][hidx(t, u - 2, v)];
+            if constexpr (u > 1) r += Rl(u - 1) * lev[n + 1][hidx(t, u - 2, v)];
           } else {
             r = Z * lev[n + 1][hidx(t, u, v - 1)];
-            if constexpr (v > 1) r += (v - 1) * lev[n + 1][hidx(t, u, v - 2)];
+            if constexpr (v > 1) r += Rl(v - 1) * lev[n + 1][hidx(t, u, v - 2)];
           }
           lev[n][hidx(t, u, v)] = r;
         });

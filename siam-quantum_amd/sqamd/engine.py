@@ -18,7 +18,7 @@ def library_path():
 
 class SqjkStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("quartets", C.c_longlong),
-                ("quartets_eval", C.c_longlong), ("launches", C.c_int)]
+                ("quartets_fp32", C.c_longlong), ("launches", C.c_int)]
 
 
 def lib():
@@ -34,6 +34,7 @@ def lib():
     L.sqjk_create.argtypes = [C.c_int, PG, PD, C.c_int, C.POINTER(C.c_void_p)]
     L.sqjk_build.argtypes = [C.c_void_p, PD, PD, C.c_double, C.c_double, C.c_int, PD, PD, PD]
     L.sqjk_schwarz.argtypes = [C.c_void_p, PD]
+    L.sqjk_set_mixed.argtypes = [C.c_void_p, C.c_double]
     L.sqjk_last_stats.argtypes = [C.c_void_p, C.POINTER(SqjkStats)]
     L.sqjk_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.sqjk_destroy.argtypes = [C.c_void_p]
@@ -76,6 +77,10 @@ class JKEngine:
         _check(lib().sqjk_build(self._h, dptr(PA), dptr(PB), fixed_cutoff, scf_cutoff,
                                 int(bool(restricted)), dptr(J), dptr(KA), dptr(KB)), "sqjk_build")
         return J, KA, KB
+
+    def set_mixed(self, threshold):
+        """Quartets with Schwarz product below `threshold` run in FP32 (0: all FP64)."""
+        _check(lib().sqjk_set_mixed(self._h, float(threshold)), "sqjk_set_mixed")
 
     def schwarz(self):
         s = np.zeros((self.n, self.n))
