@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--cutoff", type=float, default=1e-12)
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-threshold", type=float, default=1e-4,
+                    help="Schwarz product below which quartets run in FP32 for the mixed line (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -134,6 +136,23 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "flops_per_build": flops},
     }
+    if a.mixed_threshold > 0:
+        eng.set_mixed(a.mixed_threshold)
+        eng.build(P, P, a.cutoff, a.cutoff, True)
+        J64 = eng.build(P, P, a.cutoff, a.cutoff, True)  # warm, then compare below
+        t1 = time.perf_counter()
+        mk = []
+        for _ in range(a.steps):
+            Jm, Km, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
+            mk.append(eng.stats()["kernel_ms"])
+        wall_m = time.perf_counter() - t1
+        stm = eng.stats()
+        eng.set_mixed(0.0)
+        Jd, Kd, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
+        out["mixed"] = {"value": a.steps / wall_m, "unit": "J/K builds/s", "threshold": a.mixed_threshold,
+                        "kernel_ms": float(np.mean(mk)), "quartets_fp32": stm["quartets_fp32"],
+                        "max_abs_dJ": float(np.abs(Jm - Jd).max()), "max_abs_dK": float(np.abs(Km - Kd).max()),
+                        "speedup_vs_fp64": (a.steps / wall_m) / out["value"]}
     eng.close()
     if not a.no_cpu_baseline:
         try:

@@ -104,6 +104,11 @@ void GTO_JK_Matrix_Quartet_Parallel(int childID, int nBasis, const double *PA, c
                                     double fixedCutoff, double *JT, double *KA, double *KB,
                                     struct option_t *opt);
 
+/* Mixed-precision threshold used by the drop-in entry points (and so by
+ * uhf()); 0 (default) is the plain FP64 path.  No counterpart in the
+ * reference. */
+void sqjk_set_dropin_mixed(double threshold);
+
 /* src/uhf.h:26-33: zero JT, KA, KB, then one J/K build. */
 void uhf_getGMatrix(int nBasis, const struct GTOBasis_t *gto, const double *Schwarz,
                     double cutoff, const double *PA, const double *PB,

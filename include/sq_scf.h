@@ -37,6 +37,8 @@ int gen_sym_SomeEigen(int nDim, int nEigen, const double *A, const double *S, do
 int linear_solver(int nRow, int nCol, const double *A, double *B);
 /* Device eigen-solver behind the two calls above; 0 on success. */
 int sqla_gen_eigen(int n, const double *F, const double *S, double *e, double *C);
+/* P = sum_{k<nOcc} C_k C_k^T (C orbital-major) as one device DGEMM; 0 on success. */
+int sqla_density(int n, int nOcc, const double *C, double *P);
 
 /* ---- Boys function (src/fgamma.h:19-24) ---- */
 double fgamma_0(double x);

@@ -596,6 +596,12 @@ void sqjk_destroy(sqjk_ctx* c) {
 
 // ------------------------------------------------------------ drop-ins
 static sqjk_ctx* g_ctx = nullptr;  // the reference keeps this state in statics too
+static double g_dropin_mixed = 0.0;
+
+void sqjk_set_dropin_mixed(double threshold) {
+  g_dropin_mixed = threshold > 0.0 ? threshold : 0.0;
+  if (g_ctx) g_ctx->mixed_thr = g_dropin_mixed;
+}
 
 static void die(const char* where, int rc) {
   std::printf("%s - error %d: %s\n", where, rc, sqjk_errmsg());
@@ -614,6 +620,7 @@ void GTO_JK_Matrix_Quartet_Parallel(int childID, int nBasis, const double* PA, c
   if (!g_ctx) {
     int rc = sqjk_create(nBasis, gto, schwarz_basis, 0, &g_ctx);
     if (rc) die("GTO_JK_Matrix_Quartet_Parallel", rc);
+    g_ctx->mixed_thr = g_dropin_mixed;
   }
   if (opt->nCPU > 1 && childID != 0) {
     // all work is done by child 0; keep the caller's arrays in the state the

@@ -25,7 +25,7 @@ namespace {
 struct EigWork {
   int n = 0;
   rocblas_handle h = nullptr;
-  double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr;
+  double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr, *P = nullptr;
   rocblas_int* info = nullptr;
 };
 std::mutex g_mu;
@@ -34,9 +34,13 @@ EigWork g_w;
 int ensure(int n) {
   if (!g_w.h && rocblas_create_handle(&g_w.h) != rocblas_status_success) return -1;
   if (g_w.n >= n) return 0;
-  if (g_w.A) { (void)hipFree(g_w.A); (void)hipFree(g_w.B); (void)hipFree(g_w.W); (void)hipFree(g_w.E); (void)hipFree(g_w.info); }
+  if (g_w.A) {
+    (void)hipFree(g_w.A); (void)hipFree(g_w.B); (void)hipFree(g_w.W); (void)hipFree(g_w.E);
+    (void)hipFree(g_w.P); (void)hipFree(g_w.info);
+  }
   const size_t nn = (size_t)n * n;
   if (hipMalloc(&g_w.A, nn * 8) != hipSuccess || hipMalloc(&g_w.B, nn * 8) != hipSuccess ||
+      hipMalloc(&g_w.P, nn * 8) != hipSuccess ||
       hipMalloc(&g_w.W, (size_t)n * 8) != hipSuccess || hipMalloc(&g_w.E, (size_t)n * 8) != hipSuccess ||
       hipMalloc(&g_w.info, sizeof(rocblas_int)) != hipSuccess) {
     g_w.n = 0;
@@ -67,6 +71,27 @@ int sqla_gen_eigen(int n, const double* F, const double* S, double* e, double* C
   // column k of A (column-major) is orbital k: A[k*n + i]
   if (hipMemcpy(C, g_w.A, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   if (hipMemcpy(e, g_w.W, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  return 0;
+}
+
+// P = sum_{k < nOcc} C_k C_k^T with C orbital-major (C[k*n + i]); one DGEMM
+// on the device (matrix cores).  Read as column-major, C is n x n with
+// orbital k in column k, so P = C_occ C_occ^T.
+int sqla_density(int n, int nOcc, const double* C, double* P) {
+  if (n < 1 || nOcc < 0 || nOcc > n) return -1;
+  const size_t nn = (size_t)n * n;
+  if (nOcc == 0) {
+    std::memset(P, 0, nn * sizeof(double));
+    return 0;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ensure(n)) return -2;
+  if (hipMemcpy(g_w.A, C, (size_t)nOcc * n * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_transpose, n, n, nOcc, &one, g_w.A, n,
+                    g_w.A, n, &zero, g_w.P, n) != rocblas_status_success)
+    return -4;
+  if (hipMemcpy(P, g_w.P, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   return 0;
 }
 

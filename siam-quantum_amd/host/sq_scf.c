@@ -185,6 +185,15 @@ static void one_electron(int n, const struct GTOBasis_t *gto, const struct Molec
 				                GTO_moment(i, j, gto, 0, 0, 1, 0.0, 0.0, 0.0) * opt->Ez;
 }
 
+static void density_or_die(int n, int nOcc, const double *C, double *P)
+{
+	int rc = sqla_density(n, nOcc, C, P);
+	if (rc) {
+		printf("uhf: Error - device density build failed (%d)\n", rc);
+		exit(EXIT_FAILURE);
+	}
+}
+
 static void eigen_or_die(int n, int nOcc, const double *F, const double *S, double *e, double *C)
 {
 	if (nOcc == 0) return;
@@ -278,12 +287,12 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			FA[e] = H[e] + JT[e] + KA[e];
 			FB[e] = H[e] + JT[e] + KB[e];
 		}
+		/* the device eigen-solver returns S-orthonormal orbitals, so the
+		 * reference's re-normalisation (src/uhf.c:1705-1706) is a no-op here */
 		eigen_or_die(n, nEA, FA, S, eA, CA);
 		eigen_or_die(n, nEB, FB, S, eB, CB);
-		normalize_orbitals(n, nEA, S, CA);
-		normalize_orbitals(n, nEB, S, CB);
-		uhf_getDMatrix(n, nEA, CA, PA);
-		uhf_getDMatrix(n, nEB, CB, PB);
+		density_or_die(n, nEA, CA, PA);
+		density_or_die(n, nEB, CB, PB);
 		dE = Etot;
 		Etot = total_energy(n, mol, opt, PA, PB, H, JT, KA, KB);
 		dE = Etot - dE;
