@@ -29,10 +29,12 @@ int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
     return rest < 0 ? -1 : rest + 1;
   }
 }
-}  // namespace
 
 // Classes with more than 81 integrals per quartet use the cooperative
 // one-wavefront-per-quartet kernel; the others the per-thread kernel.
+// Everything here stays in the unnamed namespace: each class translation unit
+// defines these templates with a different body, so external (weak) linkage
+// would let the linker keep one class's launcher for all of them.
 constexpr bool kWave = SI::NAB * SI::NCD > 81;
 
 template <int MODE, typename R>
@@ -51,6 +53,7 @@ int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
     return launch_from<MODE, 0, R>(a, grid, st);
   }
 }
+}  // namespace
 
 int SQ_LAUNCHER(const sqjk::ClassArgs& a, int mode, int fp32, int grid, hipStream_t st) {
   switch (mode) {

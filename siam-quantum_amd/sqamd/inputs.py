@@ -193,24 +193,13 @@ def c60(r_single=1.45, r_double=1.40):
                     if i in nb[q]:
                         return True
         return False
-    # fit radius so that the mean of the two bond lengths is reproduced, then
-    # adjust radially: a symmetric cage with two bond types has each atom on
-    # one pentagon; scale pentagons about their centres to set r_single.
+    # Every atom lies on exactly one pentagon.  Shrink each pentagon about its
+    # centroid by t (bisection on the bond-length ratio), then scale the cage
+    # so the pentagon edges have length r_single.
     pent = [(i, j) for (i, j) in edges if in_pentagon(i, j)]
-    # pentagon faces: centroid of each pentagon
-    faces = []
-    seen = set()
-    for (i, j) in pent:
-        for k in nb[j]:
-            if k != i and (j, k) in pent or (k, j) in pent:
-                pass
-    # simple approach: scale so average bond is right, then solve for the
-    # pentagon shrink factor t with fixed radius by bisection
+    pent_set = set(pent)
     Pn = P / 2.0
-    ring = {}
-    for i in range(60):
-        pn = [j for j in nb[i] if (min(i, j), max(i, j)) in set(pent)]
-        ring[i] = pn
+    ring = {i: [j for j in nb[i] if (min(i, j), max(i, j)) in pent_set] for i in range(60)}
     # pentagon centroid of atom i: average of the 5-ring through i
     def pent_members(i):
         mem = {i}

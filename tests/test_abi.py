@@ -57,3 +57,21 @@ def test_struct_layouts_match_reference():
         want.append(ref["size"])
         want += [v for k, v in ref.items() if k != "size"]
     assert got == want
+
+
+def test_class_launch_helpers_have_internal_linkage():
+    """Each angular class is a separate translation unit of the same source
+    with different -DSQ_L* values.  A launch helper template with external
+    linkage would have the same mangled name and a different body in every
+    one of them, and the linker would keep one class's copy for all classes
+    (every class would run the (ss|ss) kernel).  Only the per-class entry
+    points sqjk_launch_XXXX may be exported from the launcher code."""
+    import shutil
+    import subprocess
+    import pytest
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", library_path()],
+                         capture_output=True, text=True, check=True).stdout
+    leaked = [ln for ln in out.splitlines() if re.search(r"\blaunch_(mode|wave|from)\b", ln)]
+    assert not leaked, leaked

@@ -46,12 +46,20 @@ def test_water_631gs_vs_gamess():
     assert abs(E - gms) < 2e-8
 
 
-def test_reference_defaults_three_step():
-    """Reference defaults (3-step cut-offs, SCFConv 1e-8, DIIS4) land on the
-    reference log's energy within its own noise."""
+def test_reference_defaults_three_step(oracle):
+    """Reference defaults (3-step cut-offs 1e-3/1e-6/1e-9, SCFConv 1e-8,
+    DIIS4, incremental builds).  Parity: the oracle's SCF on the same
+    protocol, to 1e-8 Eh.  The 3-step protocol itself ends 1.1e-6 Eh below
+    the tight single-step answer (GAMESS) for CH4/6-31G*, and the reference
+    log lies 1.5e-6 Eh above it, so the log is checked at 2e-6."""
     mol, bas = g2("ch4_q0m1", "631g_star")
     E = _uhf(mol, bas, 5, 5, SCFConv=1e-8)
-    assert abs(E - GOLD["ch4_q0m1"]["631g_star"]["energy"]) < 1e-6
+    Eo, _ = oracle.uhf(bas.ptr, bas.n, mol.ptr, 5, 5, scf_conv=1e-8, three_step=True,
+                       use_mpole=False, max_mem=0)
+    log = GOLD["ch4_q0m1"]["631g_star"]["energy"]
+    print(f"E(gpu) = {E:.10f}  E(oracle) = {Eo:.10f}  log = {log:.8f}")
+    assert abs(E - Eo) < 1e-8
+    assert abs(E - log) < 2e-6
 
 
 def test_uhf_doublet_radical(oracle):
