@@ -131,6 +131,10 @@ def main():
                                f"Schwarz+density screening {a.cutoff:g}, FP64",
                    "basis_functions": n, "quartets": st["quartets"],
                    "integrals": workmodel.integrals(cls)},
+        "classes": {"(%d%d|%d%d)" % lq: {"quartets": nq, "prim_quartets": npq,
+                                         "gflop": (npq * workmodel.flops_per_prim_quartet(*lq) +
+                                                   nq * workmodel.digestion_flops(*lq)) / 1e9}
+                    for lq, nq, npq in cls},
         "kernel_ms": kernel_s * 1e3,
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,

@@ -12,14 +12,22 @@
 #define SQ_CAT(a, b, c, d, e) SQ_CAT2(a, b, c, d, e)
 #define SQ_LAUNCHER SQ_CAT(sqjk_launch_, SQ_LA, SQ_LB, SQ_LC, SQ_LD)
 
+// Everything up to SQ_LAUNCHER stays in the unnamed namespace: each class
+// translation unit defines these templates with a different body, so external
+// (weak) linkage would let the linker keep one class's launcher for all.
 namespace {
 using namespace sqjk;
 constexpr int LA = SQ_LA, LB = SQ_LB, LC = SQ_LC, LD = SQ_LD;
 using SI = SliceInfo<LA, LB, LC, LD>;
 
+// Classes up to 324 integrals per quartet use the per-thread kernel (one
+// launch per ket-component slice); (dd|dp) and (dd|dd) the cooperative
+// one-wavefront-per-quartet kernel.
+constexpr bool kWave = SI::NAB * SI::NCD > 324;
+
 template <int MODE, int S, typename R>
 int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
-  if constexpr (S >= SI::NSLICE || SI::NAB * SI::NCD > 81) {
+  if constexpr (S >= SI::NSLICE || kWave) {
     return 0;
   } else {
     constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
@@ -29,13 +37,6 @@ int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
     return rest < 0 ? -1 : rest + 1;
   }
 }
-
-// Classes with more than 81 integrals per quartet use the cooperative
-// one-wavefront-per-quartet kernel; the others the per-thread kernel.
-// Everything here stays in the unnamed namespace: each class translation unit
-// defines these templates with a different body, so external (weak) linkage
-// would let the linker keep one class's launcher for all of them.
-constexpr bool kWave = SI::NAB * SI::NCD > 81;
 
 template <int MODE, typename R>
 int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {

@@ -2,9 +2,16 @@
 //
 // Work unit: one contracted shell quartet (bra pair i, ket pair j) of one
 // angular class (LA LB | LC LD), LA>=LB, LC>=LD, (LA,LB) >= (LC,LD).  In the
-// per-thread kernel below a thread evaluates a slice of the bra components
-// of that quartet for all ket components (one launch per slice) and
-// digests its integrals into J/K with atomics.
+// per-thread kernel below a lane evaluates all bra components of that quartet
+// for one slice of its ket components (one launch per slice) and digests its
+// integrals into J/K with atomics.
+//
+// Work lists are wavefront-uniform: a launch covers one class pair and one
+// precision, and is cut into segments (bra group x ket group, where a group
+// holds the pairs of one class with the same number of primitive pairs).
+// Every segment starts on a 64-slot boundary, so all lanes of a wavefront run
+// the same primitive loop counts; consecutive lanes take consecutive ket pairs
+// of one bra pair (rows of Schwarz-sorted ket prefixes).
 #pragma once
 #include "sq_md.h"
 
@@ -18,20 +25,34 @@ struct SubShell {
   int pad;
 };
 
-// unique shell pair, a = higher angular momentum
+// Unique shell pair, a = higher angular momentum.  Primitive pair v of the
+// pair is pp[pp_off + v * pp_stride]: the primitive data of a group is stored
+// primitive-major, so consecutive pairs of a group read consecutive records.
 struct PairRec {
-  int a, b;          // sub-shell indices
-  int pp_off, npp;   // primitive pairs
-  int pa, pb;        // parent reference shells of a, b
-  double sch;        // Schwarz bound of the parent pair (block maximum)
+  int a, b;               // sub-shell indices
+  int pa, pb;             // parent reference shells of a, b
+  int pp_off, pp_stride;  // primitive pairs
+  int npp, pad;
+  double sch;             // Schwarz bound of the parent pair (block maximum)
 };
 
 // primitive pair data, 48 B
-struct PrimPair {
+struct __attribute__((aligned(16))) PrimPair {
   double p;          // alpha_a + alpha_b
   double Px, Py, Pz;
   double cc;         // c_a c_b exp(-alpha_a alpha_b AB^2 / p) / p
   double eb;         //         exp(-alpha_a alpha_b AB^2 / p) / p
+};
+
+// One bra group x ket group block of a launch.  Row r (bra pair bra0 + r)
+// owns the quartets [off[row0 + r], off[row0 + r + 1]) of the segment; its
+// ket pairs are ket0 + kstart[row0 + r] + (0, 1, ...).
+struct Segment {
+  long long t0;      // first slot of the segment in the launch (multiple of 64)
+  long long count;   // quartets in the segment
+  int bra0, nrows;   // first bra pair (class-list index), rows
+  int ket0, row0;    // first ket pair of the ket group, first off[]/kstart[] entry
+  int nppb, nppk;    // primitive pairs of every bra / ket pair of the segment
 };
 
 enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
@@ -41,8 +62,9 @@ struct ClassArgs {
   const PairRec* __restrict__ bra;
   const PairRec* __restrict__ ket;
   const PrimPair* __restrict__ pp;
-  const long long* __restrict__ off;  // [nbra+1] quartet offsets per bra pair
-  const int* __restrict__ kstart;     // [nbra] first ket pair per bra pair (NULL: 0)
+  const Segment* __restrict__ seg;    // segments of this launch
+  const long long* __restrict__ off;  // per segment: nrows+1 offsets from row0
+  const int* __restrict__ kstart;     // per row: first ket within the group (NULL: 0)
   const double* __restrict__ boys;    // Boys table (global)
   const double* __restrict__ DJ;      // density for J (PA+PB)
   const double* __restrict__ DA;      // alpha density for K
@@ -53,20 +75,23 @@ struct ClassArgs {
   double* AKA;                        // K half-accumulators
   double* AKB;
   double* sch_out;                    // Schwarz mode output (n x n)
-  long long total;                    // quartets in this launch
+  long long total;                    // slots (RHF/UHF), pairs (Schwarz)
   double fixed_cut;                   // density-weighted screening cut
-  int nbra, same, n, nref;
+  int nseg, same, n, nref;
 };
 
 using namespace sqmd;
 
-// slice geometry: bra components per thread
+// Ket-component slices: a lane holds all NAB bra components for SZ ket
+// components (at most 36 integrals, at least one ket component).  Slicing the
+// ket side keeps the W intermediates of different slices disjoint.
 template <int LA, int LB, int LC, int LD>
 struct SliceInfo {
   static constexpr int NAB = ncart(LA) * ncart(LB);
   static constexpr int NCD = ncart(LC) * ncart(LD);
-  static constexpr int SZ = (36 / NCD) > 0 ? ((36 / NCD) < NAB ? (36 / NCD) : NAB) : 1;
-  static constexpr int NSLICE = (NAB + SZ - 1) / SZ;
+  static constexpr int PER = (36 / NAB) > 0 ? (36 / NAB) : 1;
+  static constexpr int NSLICE = (NCD + PER - 1) / PER;
+  static constexpr int SZ = (NCD + NSLICE - 1) / NSLICE;
 };
 
 __host__ __device__ __forceinline__ void fatomic(double* p, double v) {
@@ -77,30 +102,61 @@ __host__ __device__ __forceinline__ void fatomic(double* p, double v) {
 #endif
 }
 
-// R: precision of the integral evaluation (double, or float for the
-// mixed-precision range); digestion and accumulation are always FP64.
-template <int LA, int LB, int LC, int LD, int AB0, int AB1, int MODE, typename R = double>
-__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, const double* lds) {
+// last segment whose first slot is <= t
+__host__ __device__ __forceinline__ int seg_of(const Segment* __restrict__ seg, int nseg, long long t) {
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg[mid].t0 <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// row r with off[r] <= q < off[r+1], for 0 <= q < off[nrows]
+__host__ __device__ __forceinline__ int row_of(const long long* __restrict__ off, int nrows, long long q) {
+  int lo = 0, hi = nrows;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= q) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Sum over the 64 lanes of the wavefront (all lanes active).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Contracted quartet (ib, ik), ket components [CD0, CD1).  R: precision of
+// the integral evaluation (double, or float for the mixed-precision range);
+// digestion and accumulation are FP64.  All lanes of a wavefront call this
+// together; `valid` gates the lane's contributions, so cross-lane reductions
+// see every lane.
+template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE, typename R = double>
+__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, int nppb, int nppk,
+                                                 bool valid, const double* lds) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
-  constexpr int NS = AB1 - AB0, NCD = NC * ND;
+  constexpr int LAB = LA + LB, L = LAB + LC + LD;
+  constexpr int NAB = NA * NB, NS = CD1 - CD0;
 
   const PairRec bp = A.bra[ib];
   const PairRec kp = A.ket[ik];
   const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
 
-  R acc[NS][NCD];
-  sfor<NS>([&](auto s) { sfor<NCD>([&](auto c) { acc[s][c] = R(0); }); });
+  R acc[NAB][NS];
+  sfor<NAB>([&](auto a) { sfor<NS>([&](auto c) { acc[a][c] = R(0); }); });
 
-  for (int u = 0; u < bp.npp; ++u) {
-    const PrimPair P = A.pp[bp.pp_off + u];
+  for (int u = 0; u < nppb; ++u) {
+    const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
     const R h2p = R(0.5 / P.p);
     Herm1<LA, LB, R> Ex, Ey, Ez;
     herm_coef<LA, LB, R>(Ex, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
     herm_coef<LA, LB, R>(Ey, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
     herm_coef<LA, LB, R>(Ez, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
-    for (int v = 0; v < kp.npp; ++v) {
-      const PrimPair Q = A.pp[kp.pp_off + v];
+    for (int v = 0; v < nppk; ++v) {
+      const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
       const double s = P.p + Q.p;
       const double rs = 1.0 / sqrt(s);
       if constexpr (MODE != kSchwarz) {
@@ -120,8 +176,9 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       herm_coef<LC, LD, R>(Fz, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
       const R pref = R(kTwoPi25 * P.cc * Q.cc * rs);
 
-      sfor<NCD>([&](auto cdI) {
-        constexpr int cd = decltype(cdI)::value;
+      sfor<NS>([&](auto csI) {
+        constexpr int cs = decltype(csI)::value;
+        constexpr int cd = CD0 + cs;
         constexpr Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
         constexpr int kx = c.x + d.x, ky = c.y + d.y, kz = c.z + d.z;
         // W[t][u][v] = sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V}
@@ -150,8 +207,8 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
             });
           });
         });
-        sfor<NS>([&](auto sI) {
-          constexpr int ab = AB0 + decltype(sI)::value;
+        sfor<NAB>([&](auto abI) {
+          constexpr int ab = decltype(abI)::value;
           constexpr Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
           R val = R(0);
           sfor<a.x + b.x + 1>([&](auto tI) {
@@ -166,7 +223,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
               val += Ex.e[a.x][b.x][t] * Ey.e[a.y][b.y][uu] * vz;
             });
           });
-          acc[decltype(sI)::value][cd] += pref * val;
+          acc[ab][cs] += pref * val;
         });
       });
     }
@@ -177,103 +234,118 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
 
   if constexpr (MODE == kSchwarz) {
     // diagonal (ab|ab) of the quartet (i,i): sqrt|.| into both (a,b), (b,a)
-    sfor<NS>([&](auto sI) {
-      constexpr int ab = AB0 + decltype(sI)::value;
+    sfor<NS>([&](auto csI) {
+      constexpr int cs = decltype(csI)::value;
+      constexpr int ab = CD0 + cs;
       constexpr int ia = ab / NB, jb = ab % NB;
       const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
-      const double v = sqrt(fabs((double)acc[decltype(sI)::value][ab] * sc * sc));
+      const double v = sqrt(fabs((double)acc[ab][cs] * sc * sc));
       A.sch_out[(fa + ia) * n + fb + jb] = v;
       A.sch_out[(fb + jb) * n + fa + ia] = v;
     });
     return;
   } else {
-    double deg = 1.0;
+    double deg = valid ? 1.0 : 0.0;
     if (bp.a == bp.b) deg *= 0.5;
     if (kp.a == kp.b) deg *= 0.5;
     if (A.same && ib == ik) deg *= 0.5;
 
-    double sc_c[NC], sc_d[ND];
-    sfor<NC>([&](auto I) { sc_c[decltype(I)::value] = A.bscale[fc + decltype(I)::value]; });
-    sfor<ND>([&](auto I) { sc_d[decltype(I)::value] = A.bscale[fd + decltype(I)::value]; });
+    double sc_a[NA], sc_b[NB];
+    sfor<NA>([&](auto I) { sc_a[I.value] = A.bscale[fa + I.value]; });
+    sfor<NB>([&](auto I) { sc_b[I.value] = A.bscale[fb + I.value]; });
 
     // scaled integrals, FP64 from here on
-    double vv_[NS][NCD];
-    sfor<NS>([&](auto sI) {
-      constexpr int ab = AB0 + decltype(sI)::value;
-      const double sab = deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB];
-      sfor<NCD>([&](auto cI) {
-        constexpr int cd = decltype(cI)::value;
-        vv_[decltype(sI)::value][cd] = (double)acc[decltype(sI)::value][cd] * sab * sc_c[cd / ND] * sc_d[cd % ND];
+    double vv_[NAB][NS];
+    sfor<NS>([&](auto csI) {
+      constexpr int cs = decltype(csI)::value;
+      constexpr int cd = CD0 + cs;
+      const double scd = deg * A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
+      sfor<NAB>([&](auto abI) {
+        constexpr int ab = decltype(abI)::value;
+        vv_[ab][cs] = (double)acc[ab][cs] * scd * sc_a[ab / NB] * sc_b[ab % NB];
       });
     });
 
     // Coulomb: A_J[a,b] += 2 sum_cd v D[c,d];  A_J[c,d] += 2 sum_ab v D[a,b]
-    double dcd[NCD];
-    sfor<NCD>([&](auto cI) {
-      constexpr int cd = decltype(cI)::value;
-      dcd[cd] = A.DJ[(fc + cd / ND) * n + fd + cd % ND];
+    double dcd[NS], jcd[NS], jab[NAB];
+    sfor<NS>([&](auto csI) {
+      constexpr int cd = CD0 + decltype(csI)::value;
+      dcd[csI.value] = A.DJ[(fc + cd / ND) * n + fd + cd % ND];
+      jcd[csI.value] = 0.0;
     });
-    double jcd[NCD];
-    sfor<NCD>([&](auto cI) { jcd[decltype(cI)::value] = 0.0; });
-    sfor<NS>([&](auto sI) {
-      constexpr int ab = AB0 + decltype(sI)::value;
-      constexpr int s = decltype(sI)::value;
+    sfor<NAB>([&](auto abI) {
+      constexpr int ab = decltype(abI)::value;
       const double dab = A.DJ[(fa + ab / NB) * n + fb + ab % NB];
-      double jab = 0.0;
-      sfor<NCD>([&](auto cI) {
-        constexpr int cd = decltype(cI)::value;
-        jab += vv_[s][cd] * dcd[cd];
-        jcd[cd] += vv_[s][cd] * dab;
+      double s = 0.0;
+      sfor<NS>([&](auto csI) {
+        constexpr int cs = decltype(csI)::value;
+        s += vv_[ab][cs] * dcd[cs];
+        jcd[cs] += vv_[ab][cs] * dab;
       });
-      fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * jab);
+      jab[ab] = s;
     });
-    sfor<NCD>([&](auto cI) {
-      constexpr int cd = decltype(cI)::value;
-      fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * jcd[cd]);
-    });
+    bool bra_uniform = false;
+#ifdef __HIP_DEVICE_COMPILE__
+    // all lanes on one bra pair (the common case): one atomic per J_ab element
+    bra_uniform = __all(ib == __builtin_amdgcn_readfirstlane(ib));
+    if (bra_uniform) {
+      const bool lead = (threadIdx.x & 63) == 0;
+      sfor<NAB>([&](auto abI) {
+        constexpr int ab = decltype(abI)::value;
+        const double s = wave_sum(jab[ab]);
+        if (lead) fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * s);
+      });
+    }
+#endif
+    if (valid) {
+      if (!bra_uniform)
+        sfor<NAB>([&](auto abI) {
+          constexpr int ab = decltype(abI)::value;
+          fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * jab[ab]);
+        });
+      sfor<NS>([&](auto csI) {
+        constexpr int cd = CD0 + decltype(csI)::value;
+        fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * jcd[csI.value]);
+      });
+    }
 
     // Exchange (per spin): A_K[a,c] += v D[b,d]; A_K[b,c] += v D[a,d];
     //                      A_K[a,d] += v D[b,c]; A_K[b,d] += v D[a,c]
     auto exchange = [&](const double* __restrict__ D, double* AK) {
-      constexpr int A0 = AB0 / NB, A1 = (AB1 - 1) / NB;  // bra a-range in slice
-      constexpr int NAr = A1 - A0 + 1;
-      constexpr int B0 = (NAr > 1) ? 0 : AB0 % NB;
-      constexpr int B1 = (NAr > 1) ? NB - 1 : (AB1 - 1) % NB;
-      constexpr int NBr = B1 - B0 + 1;
-      double kac[NAr][NC], kad[NAr][ND], kbc[NBr][NC], kbd[NBr][ND];
-      sfor<NAr>([&](auto I) { sfor<NC>([&](auto J) { kac[I.value][J.value] = 0.0; });
-                              sfor<ND>([&](auto J) { kad[I.value][J.value] = 0.0; }); });
-      sfor<NBr>([&](auto I) { sfor<NC>([&](auto J) { kbc[I.value][J.value] = 0.0; });
-                              sfor<ND>([&](auto J) { kbd[I.value][J.value] = 0.0; }); });
-      sfor<NS>([&](auto sI) {
-        constexpr int s = decltype(sI)::value;
-        constexpr int ab = AB0 + s;
-        constexpr int ia = ab / NB, jb = ab % NB;
-        const int ga = fa + ia, gb = fb + jb;
-        sfor<NC>([&](auto cI) {
-          constexpr int ic_ = decltype(cI)::value;
-          const int gc = fc + ic_;
-          const double dbc = D[gb * n + gc], dac = D[ga * n + gc];
-          sfor<ND>([&](auto dI) {
-            constexpr int id = decltype(dI)::value;
-            const int gd = fd + id;
-            const double v = vv_[s][ic_ * ND + id];
-            kac[ia - A0][ic_] += v * D[gb * n + gd];
-            kbc[jb - B0][ic_] += v * D[ga * n + gd];
-            kad[ia - A0][id] += v * dbc;
-            kbd[jb - B0][id] += v * dac;
-          });
+      constexpr int C0 = CD0 / ND, C1 = (CD1 - 1) / ND;  // ket c-range of the slice
+      constexpr int NCr = C1 - C0 + 1;
+      constexpr int D0 = (NCr > 1) ? 0 : CD0 % ND;
+      constexpr int D1 = (NCr > 1) ? ND - 1 : (CD1 - 1) % ND;
+      constexpr int NDr = D1 - D0 + 1;
+      double kac[NA][NCr], kad[NA][NDr], kbc[NB][NCr], kbd[NB][NDr];
+      sfor<NA>([&](auto I) { sfor<NCr>([&](auto J) { kac[I.value][J.value] = 0.0; });
+                             sfor<NDr>([&](auto J) { kad[I.value][J.value] = 0.0; }); });
+      sfor<NB>([&](auto I) { sfor<NCr>([&](auto J) { kbc[I.value][J.value] = 0.0; });
+                             sfor<NDr>([&](auto J) { kbd[I.value][J.value] = 0.0; }); });
+      sfor<NS>([&](auto csI) {
+        constexpr int cs = decltype(csI)::value;
+        constexpr int cd = CD0 + cs;
+        constexpr int ic = cd / ND, id = cd % ND;
+        const int gc = fc + ic, gd = fd + id;
+        sfor<NAB>([&](auto abI) {
+          constexpr int ab = decltype(abI)::value;
+          constexpr int ia = ab / NB, jb = ab % NB;
+          const int ga = fa + ia, gb = fb + jb;
+          const double v = vv_[ab][cs];
+          kac[ia][ic - C0] += v * D[gb * n + gd];
+          kbc[jb][ic - C0] += v * D[ga * n + gd];
+          kad[ia][id - D0] += v * D[gb * n + gc];
+          kbd[jb][id - D0] += v * D[ga * n + gc];
         });
       });
-      sfor<NAr>([&](auto I) {
-        constexpr int ia = A0 + decltype(I)::value;
-        sfor<NC>([&](auto J) { fatomic(&AK[(fa + ia) * n + fc + J.value], kac[I.value][J.value]); });
-        sfor<ND>([&](auto J) { fatomic(&AK[(fa + ia) * n + fd + J.value], kad[I.value][J.value]); });
+      if (!valid) return;
+      sfor<NA>([&](auto I) {
+        sfor<NCr>([&](auto J) { fatomic(&AK[(fa + I.value) * n + fc + C0 + J.value], kac[I.value][J.value]); });
+        sfor<NDr>([&](auto J) { fatomic(&AK[(fa + I.value) * n + fd + D0 + J.value], kad[I.value][J.value]); });
       });
-      sfor<NBr>([&](auto I) {
-        constexpr int jb = B0 + decltype(I)::value;
-        sfor<NC>([&](auto J) { fatomic(&AK[(fb + jb) * n + fc + J.value], kbc[I.value][J.value]); });
-        sfor<ND>([&](auto J) { fatomic(&AK[(fb + jb) * n + fd + J.value], kbd[I.value][J.value]); });
+      sfor<NB>([&](auto I) {
+        sfor<NCr>([&](auto J) { fatomic(&AK[(fb + I.value) * n + fc + C0 + J.value], kbc[I.value][J.value]); });
+        sfor<NDr>([&](auto J) { fatomic(&AK[(fb + I.value) * n + fd + D0 + J.value], kbd[I.value][J.value]); });
       });
     };
     exchange(A.DA, A.AKA);
@@ -281,44 +353,57 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
   }
 }
 
-// quartet index -> (bra pair, ket pair) through the per-bra offsets
-__device__ __forceinline__ int find_bra(const long long* __restrict__ off, int nbra, long long t) {
-  int lo = 0, hi = nbra;  // off[lo] <= t < off[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= t) lo = mid; else hi = mid;
-  }
-  return lo;
+// Density-weighted screening at the reference-shell level (the reference's
+// six-block sum and test, src/matrix.c:3921-3924, 3936, 3943).
+__host__ __device__ __forceinline__ bool density_screen_pass(const ClassArgs& A, const PairRec& bp,
+                                                             const PairRec& kp) {
+  const double* PM = A.PM;
+  const int nr = A.nref;
+  const double mp = PM[bp.pa * nr + bp.pb] + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
+                    PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
+  return !(bp.sch * kp.sch * mp < A.fixed_cut);
 }
 
-// One launch per (class, mode, slice S).  Grid-stride over the quartets of
-// the class; each workgroup first copies the Boys rows it needs into LDS.
+// One launch per (class pair, precision, mode, slice S).  RHF/UHF: grid-stride
+// over wavefronts; a wavefront takes 64 consecutive slots of one segment.
+// Schwarz: grid-stride over the pairs of the class, diagonal quartets.
 template <int LA, int LB, int LC, int LD, int MODE, int S, typename R = double>
 __global__ void __launch_bounds__(256) k_class(ClassArgs A) {
   using SI = SliceInfo<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
-  constexpr int AB0 = S * SI::SZ;
-  constexpr int AB1 = (AB0 + SI::SZ < SI::NAB) ? AB0 + SI::SZ : SI::NAB;
+  constexpr int CD0 = S * SI::SZ;
+  constexpr int CD1 = (CD0 + SI::SZ < SI::NCD) ? CD0 + SI::SZ : SI::NCD;
+  static_assert(CD0 < CD1, "empty ket-component slice");
   extern __shared__ double lds[];
   boys_load_lds<L>(lds, A.boys);
   __syncthreads();
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
-    int ib, ik;
-    if constexpr (MODE == kSchwarz) {
-      ib = ik = (int)t;
-    } else {
-      ib = find_bra(A.off, A.nbra, t);
-      ik = (int)(t - A.off[ib]) + (A.kstart ? A.kstart[ib] : 0);
-      // density-weighted screening at the reference-shell level
-      const PairRec bp = A.bra[ib], kp = A.ket[ik];
-      const double* PM = A.PM;
-      const int nr = A.nref;
-      const double mp = PM[bp.pa * nr + bp.pb] + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
-                        PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
-      if (bp.sch * kp.sch * mp < A.fixed_cut) continue;
+  if constexpr (MODE == kSchwarz) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
+      const int npp = A.bra[t].npp;
+      quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, npp, true, lds);
     }
-    quartet<LA, LB, LC, LD, AB0, AB1, MODE, R>(A, ib, ik, lds);
+  } else {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const long long nw = A.total >> 6;
+    const long long wstride = (long long)gridDim.x * wpb;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (long long w = (long long)blockIdx.x * wpb + wib; w < nw; w += wstride) {
+      const long long t0 = w << 6;
+      const Segment sg = A.seg[seg_of(A.seg, A.nseg, t0)];
+      const long long* off = A.off + sg.row0;
+      const long long q0 = t0 - sg.t0;  // < sg.count: segments start on 64-slot boundaries
+      int r = row_of(off, sg.nrows, q0);
+      const long long q = q0 + lane;
+      const bool in = q < sg.count;
+      if (in)
+        while (off[r + 1] <= q) ++r;
+      const int ib = sg.bra0 + r;
+      const int ik = in ? sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]) : sg.ket0;
+      const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
+      quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, ib, ik, sg.nppb, sg.nppk, valid, lds);
+    }
   }
 }
 

@@ -10,10 +10,14 @@
 //   * Schwarz bounds sqrt|(pq|pq)| computed on the device unless given;
 //   * unique sub-shell pairs binned by pair class (ss,ps,pp,ds,dp,dd) and
 //     sorted by the parent-level Schwarz bound, descending.
+//   * within a class, pairs with the same number of primitive pairs form a
+//     group (sorted by Schwarz bound inside the group) whose primitive data is
+//     stored primitive-major.
 // Per build (sqjk_build):
-//   * for every class pair (X>=Y) the surviving quartets of bra pair i are
-//     the prefix j < cnt_i of the sorted ket list; offsets are uploaded and
-//     each class kernel walks its quartets with a grid-stride loop;
+//   * for every class pair (X>=Y), bra group and ket group, the surviving
+//     quartets of bra pair i are the prefix j < cnt_i of the ket group; the
+//     rows and their offsets (a segment) depend only on the cut-offs and are
+//     kept on the device until the cut-offs change;
 //   * integrals are digested into half-accumulators A (J = A + A^T) with
 //     atomics; a finalize kernel forms J, -K and the host applies the
 //     reference's accumulate/restricted/symmetrise semantics.
@@ -134,11 +138,21 @@ struct sqjk_ctx {
   double *d_DJ = nullptr, *d_DA = nullptr, *d_DB = nullptr, *d_PM = nullptr;
   double *d_AJ = nullptr, *d_AKA = nullptr, *d_AKB = nullptr;
   double *d_J = nullptr, *d_KA = nullptr, *d_KB = nullptr, *d_sch = nullptr;
-  long long* d_off = nullptr;
-  int* d_kstart = nullptr;
-  size_t off_cap = 0, ks_cap = 0;
+  struct Group { int begin, count, npp; };
+  std::vector<Group> groups[kNPairClass];
+  // work plan for the cut-offs (thr, thr64) of the last build
+  struct Launch { int X, Y, fp32, seg0, nseg; long long slots, quartets; };
+  bool plan_ok = false;
+  double plan_thr = 0.0, plan_thr64 = 0.0;
+  std::vector<Launch> launches;
+  std::vector<Segment> h_seg;
   std::vector<long long> h_off;
-  std::vector<int> h_kstart;
+  std::vector<int> h_ks;
+  Segment* d_seg = nullptr;
+  long long* d_off = nullptr;
+  int* d_ks = nullptr;
+  size_t seg_cap = 0, off_cap = 0, ks_cap = 0;
+  long long plan_q = 0, plan_q32 = 0;
   double mixed_thr = 0.0;  // Schwarz products below this run in FP32
   std::vector<double> h_DJ, h_PM, h_J, h_KA, h_KB;
   sqjk_stats last{};
@@ -238,7 +252,7 @@ void plan_pairs(sqjk_ctx* c, const std::vector<double>& pexp, const std::vector<
       const SubShell &A = c->sub[a], &B = c->sub[b];
       PairRec pr{};
       pr.a = a; pr.b = b; pr.pa = A.parent; pr.pb = B.parent;
-      pr.pp_off = (int)c->pp.size(); pr.npp = A.nprim * B.nprim;
+      pr.pp_off = (int)c->pp.size(); pr.pp_stride = 1; pr.npp = A.nprim * B.nprim;
       const double ab2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
       for (int ka = 0; ka < A.nprim; ++ka)
         for (int kb = 0; kb < B.nprim; ++kb) {
@@ -310,7 +324,6 @@ int compute_schwarz(sqjk_ctx* c) {
     ClassArgs a = base_args(c);
     a.bra = a.ket = c->d_pairs[X];
     a.total = (long long)c->pairs[X].size();
-    a.nbra = (int)c->pairs[X].size();
     a.same = 1;
     ClassLauncher f = launcher_for(kPairL[X][0], kPairL[X][1], kPairL[X][0], kPairL[X][1]);
     if (!f || f(a, kSchwarz, 0, grid_for(a.total, 256), c->st) < 0)
@@ -333,11 +346,129 @@ void finish_pairs(sqjk_ctx* c) {
           r = std::max(r, c->sch_basis[(size_t)p * n + q]);
       c->sch_ref[(size_t)P * nr + Q] = r;
     }
+  // groups: most primitive pairs first, Schwarz bound descending inside
+  std::vector<PrimPair> ppT;
+  ppT.reserve(c->pp.size());
   for (int X = 0; X < kNPairClass; ++X) {
-    for (PairRec& pr : c->pairs[X]) pr.sch = c->sch_ref[(size_t)pr.pa * nr + pr.pb];
-    std::stable_sort(c->pairs[X].begin(), c->pairs[X].end(),
-                     [](const PairRec& x, const PairRec& y) { return x.sch > y.sch; });
+    auto& prs = c->pairs[X];
+    for (PairRec& pr : prs) pr.sch = c->sch_ref[(size_t)pr.pa * nr + pr.pb];
+    std::stable_sort(prs.begin(), prs.end(), [](const PairRec& x, const PairRec& y) {
+      return x.npp != y.npp ? x.npp > y.npp : x.sch > y.sch;
+    });
+    c->groups[X].clear();
+    for (int i = 0; i < (int)prs.size();) {
+      int j = i;
+      while (j < (int)prs.size() && prs[j].npp == prs[i].npp) ++j;
+      c->groups[X].push_back({i, j - i, prs[i].npp});
+      // primitive-major copy: record (v, pair) at base + v * count + pair
+      const int base = (int)ppT.size(), cnt = j - i;
+      ppT.resize(ppT.size() + (size_t)cnt * prs[i].npp);
+      for (int k = 0; k < cnt; ++k) {
+        PairRec& pr = prs[i + k];
+        for (int v = 0; v < pr.npp; ++v) ppT[(size_t)base + (size_t)v * cnt + k] = c->pp[pr.pp_off + v];
+        pr.pp_off = base + k;
+        pr.pp_stride = cnt;
+      }
+      i = j;
+    }
   }
+  c->pp.swap(ppT);
+  c->plan_ok = false;
+}
+
+// Rows and segments of every launch for the cut-offs (thr, thr64): quartets
+// with Schwarz product >= thr64 run in FP64, those in [thr, thr64) in FP32.
+// Host only; make_plan uploads the result.
+void build_plan(sqjk_ctx* c, double thr, double thr64) {
+  c->launches.clear();
+  c->h_seg.clear();
+  c->h_off.clear();
+  c->h_ks.clear();
+  std::memset(c->cls_q, 0, sizeof(c->cls_q));
+  std::memset(c->cls_p, 0, sizeof(c->cls_p));
+  c->plan_q = c->plan_q32 = 0;
+  std::vector<long long> offs;
+  std::vector<int> kss;
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y <= X; ++Y)
+      for (int fp = 0; fp < 2; ++fp) {
+        if (fp == 1 && !(thr64 > thr)) continue;
+        const auto &bra = c->pairs[X], &ket = c->pairs[Y];
+        sqjk_ctx::Launch L{X, Y, fp, (int)c->h_seg.size(), 0, 0, 0};
+        for (size_t a = 0; a < c->groups[X].size(); ++a)
+          for (size_t b = 0; b < c->groups[Y].size(); ++b) {
+            if (X == Y && b > a) continue;  // canonical: ket pair index <= bra pair index
+            const auto &g1 = c->groups[X][a], &g2 = c->groups[Y][b];
+            offs.clear();
+            kss.clear();
+            long long acc = 0;
+            // bra and ket groups are sorted by sch, so the ket prefix that
+            // passes can only shrink as the bra index grows: one sweep
+            int lo = g2.count, lo64 = g2.count;
+            for (int i = 0; i < g1.count; ++i) {
+              const double si = bra[g1.begin + i].sch;
+              while (lo > 0 && !(si * ket[g2.begin + lo - 1].sch >= thr)) --lo;
+              while (lo64 > 0 && !(si * ket[g2.begin + lo64 - 1].sch >= thr64)) --lo64;
+              int cnt = lo, c64 = std::min(lo64, lo);
+              if (X == Y && a == b) { cnt = std::min(cnt, i + 1); c64 = std::min(c64, i + 1); }
+              if (cnt == 0) break;
+              const int k0 = fp ? c64 : 0, k1 = fp ? cnt : c64;
+              offs.push_back(acc);
+              kss.push_back(k0);
+              acc += k1 - k0;
+            }
+            while (!offs.empty() && offs.back() == acc) { offs.pop_back(); kss.pop_back(); }
+            if (acc == 0) continue;
+            Segment sg{};
+            sg.t0 = L.slots;
+            sg.count = acc;
+            sg.bra0 = g1.begin;
+            sg.nrows = (int)offs.size();
+            sg.ket0 = g2.begin;
+            sg.row0 = (int)c->h_off.size();
+            sg.nppb = g1.npp;
+            sg.nppk = g2.npp;
+            c->h_seg.push_back(sg);
+            c->h_off.insert(c->h_off.end(), offs.begin(), offs.end());
+            c->h_off.push_back(acc);
+            c->h_ks.insert(c->h_ks.end(), kss.begin(), kss.end());
+            c->h_ks.push_back(0);
+            L.slots += (acc + 63) / 64 * 64;
+            L.quartets += acc;
+            L.nseg++;
+            c->cls_p[X][Y] += acc * g1.npp * g2.npp;
+          }
+        if (L.nseg == 0) continue;
+        c->launches.push_back(L);
+        c->cls_q[X][Y] += L.quartets;
+        c->plan_q += L.quartets;
+        if (fp) c->plan_q32 += L.quartets;
+      }
+}
+
+int make_plan(sqjk_ctx* c, double thr, double thr64) {
+  if (c->plan_ok && c->plan_thr == thr && c->plan_thr64 == thr64) return SQJK_OK;
+  build_plan(c, thr, thr64);
+  if (c->h_off.size() > (size_t)INT32_MAX) return fail(SQJK_ENOMEM, "work plan too large");
+  auto grow = [&](auto** d, size_t& cap, size_t need) -> int {
+    if (cap >= need && *d) return SQJK_OK;
+    if (*d) { (void)hipFree(*d); *d = nullptr; }
+    cap = std::max<size_t>(need, 1024) * 3 / 2;
+    return dalloc(d, cap);
+  };
+  int rc;
+  if ((rc = grow(&c->d_seg, c->seg_cap, c->h_seg.size()))) return rc;
+  if ((rc = grow(&c->d_off, c->off_cap, c->h_off.size()))) return rc;
+  if ((rc = grow(&c->d_ks, c->ks_cap, c->h_ks.size()))) return rc;
+  if (!c->h_seg.empty()) {
+    HIPCHK(hipMemcpyAsync(c->d_seg, c->h_seg.data(), c->h_seg.size() * sizeof(Segment), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_off, c->h_off.data(), c->h_off.size() * sizeof(long long), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_ks, c->h_ks.data(), c->h_ks.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+  }
+  c->plan_ok = true;
+  c->plan_thr = thr;
+  c->plan_thr64 = thr64;
+  return SQJK_OK;
 }
 
 }  // namespace
@@ -375,6 +506,12 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   }
   finish_pairs(c);
   if ((rc = upload_pairs(c))) { sqjk_destroy(c); return rc; }
+  // primitive data in the grouped, primitive-major order (same size)
+  if (hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st) !=
+      hipSuccess) {
+    sqjk_destroy(c);
+    return fail(SQJK_EHIP, "primitive pair upload failed");
+  }
   if (hipStreamSynchronize(c->st) != hipSuccess) { sqjk_destroy(c); return fail(SQJK_EHIP, "sync failed"); }
   *out = c;
   return SQJK_OK;
@@ -409,80 +546,11 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   if (uhf) HIPCHK(hipMemcpyAsync(c->d_DB, PB, nn * 8, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_PM, c->h_PM.data(), (size_t)nr * nr * 8, hipMemcpyHostToDevice, c->st));
 
-  // surviving quartet ranges per class pair: ket pairs [0, c64) in FP64 and
-  // [c64, cnt) in FP32, where c64 counts Schwarz products >= mixed_thr
+  // work plan (rows/segments), rebuilt only when the cut-offs change
   const double thr = std::max(fixedCutoff, scfCutoff);
   const double thr64 = std::max(thr, c->mixed_thr);
-  struct Launch { int X, Y, fp32; size_t off, ks; long long total; };
-  std::vector<Launch> plan;
-  c->h_off.clear();
-  c->h_kstart.clear();
-  long long nq = 0, nq32 = 0;
-  std::memset(c->cls_q, 0, sizeof(c->cls_q));
-  std::memset(c->cls_p, 0, sizeof(c->cls_p));
-  std::vector<long long> kpre;
-  std::vector<size_t> c64v;
-  for (int X = 0; X < kNPairClass; ++X)
-    for (int Y = 0; Y <= X; ++Y) {
-      const auto &bra = c->pairs[X], &ket = c->pairs[Y];
-      if (bra.empty() || ket.empty()) continue;
-      const size_t base = c->h_off.size();
-      long long acc = 0, prim = 0;
-      kpre.assign(ket.size() + 1, 0);  // prefix sums of ket primitive pairs
-      for (size_t j = 0; j < ket.size(); ++j) kpre[j + 1] = kpre[j] + ket[j].npp;
-      c64v.assign(bra.size(), 0);
-      // both lists are sorted by sch descending, so the number of ket pairs
-      // with sch_i*sch_j >= thr can only shrink as i grows: one sweep
-      size_t lo = ket.size(), lo64 = ket.size();
-      for (size_t i = 0; i < bra.size(); ++i) {
-        c->h_off.push_back(acc);
-        while (lo > 0 && !(bra[i].sch * ket[lo - 1].sch >= thr)) --lo;
-        while (lo64 > 0 && !(bra[i].sch * ket[lo64 - 1].sch >= thr64)) --lo64;
-        size_t cnt = lo, c64 = lo64;
-        if (X == Y) { cnt = std::min(cnt, i + 1); c64 = std::min(c64, i + 1); }
-        c64v[i] = c64;
-        acc += (long long)c64;
-        prim += (long long)bra[i].npp * kpre[cnt];
-      }
-      c->h_off.push_back(acc);
-      if (acc > 0) plan.push_back({X, Y, 0, base, 0, acc});
-      // FP32 range
-      const size_t base32 = c->h_off.size(), ks = c->h_kstart.size();
-      long long acc32 = 0;
-      lo = ket.size();
-      for (size_t i = 0; i < bra.size(); ++i) {
-        c->h_off.push_back(acc32);
-        c->h_kstart.push_back((int)c64v[i]);
-        while (lo > 0 && !(bra[i].sch * ket[lo - 1].sch >= thr)) --lo;
-        size_t cnt = lo;
-        if (X == Y) cnt = std::min(cnt, i + 1);
-        acc32 += (long long)(cnt > c64v[i] ? cnt - c64v[i] : 0);
-      }
-      c->h_off.push_back(acc32);
-      if (acc32 > 0) plan.push_back({X, Y, 1, base32, ks, acc32});
-      c->cls_q[X][Y] = acc + acc32;
-      c->cls_p[X][Y] = prim;
-      nq += acc + acc32;
-      nq32 += acc32;
-    }
-  if (c->off_cap < c->h_off.size()) {
-    if (c->d_off) (void)hipFree(c->d_off);
-    c->off_cap = std::max<size_t>(c->h_off.size(), 1024) * 2;
-    int rc = dalloc(&c->d_off, c->off_cap);
-    if (rc) return rc;
-  }
-  if (c->ks_cap < c->h_kstart.size() || !c->d_kstart) {
-    if (c->d_kstart) (void)hipFree(c->d_kstart);
-    c->ks_cap = std::max<size_t>(c->h_kstart.size(), 1024) * 2;
-    int rc = dalloc(&c->d_kstart, c->ks_cap);
-    if (rc) return rc;
-  }
-  if (!c->h_off.empty())
-    HIPCHK(hipMemcpyAsync(c->d_off, c->h_off.data(), c->h_off.size() * sizeof(long long),
-                          hipMemcpyHostToDevice, c->st));
-  if (!c->h_kstart.empty())
-    HIPCHK(hipMemcpyAsync(c->d_kstart, c->h_kstart.data(), c->h_kstart.size() * sizeof(int),
-                          hipMemcpyHostToDevice, c->st));
+  int rc = make_plan(c, thr, thr64);
+  if (rc) return rc;
   const int fill_grid = grid_for((long long)nn, 256);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, nn, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AKA, nn, 0.0);
@@ -490,18 +558,19 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev0, c->st));
   int launches = 0;
-  for (const Launch& L : plan) {
+  for (const sqjk_ctx::Launch& L : c->launches) {
     ClassArgs a = base_args(c);
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
-    a.off = c->d_off + L.off;
-    a.kstart = L.fp32 ? c->d_kstart + L.ks : nullptr;
-    a.nbra = (int)c->pairs[L.X].size();
+    a.seg = c->d_seg + L.seg0;
+    a.nseg = L.nseg;
+    a.off = c->d_off;
+    a.kstart = L.fp32 ? c->d_ks : nullptr;
     a.same = (L.X == L.Y);
-    a.total = L.total;
+    a.total = L.slots;
     a.fixed_cut = fixedCutoff;
     ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
-    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, grid_for(L.total, 256), c->st) : -1;
+    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, grid_for(L.slots, 256), c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
   }
@@ -531,8 +600,8 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last.kernel_ms = ms;
   c->last.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  c->last.quartets = nq;
-  c->last.quartets_fp32 = nq32;
+  c->last.quartets = c->plan_q;
+  c->last.quartets_fp32 = c->plan_q32;
   c->last.launches = launches;
   return SQJK_OK;
 }
@@ -583,7 +652,7 @@ void sqjk_destroy(sqjk_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   void* ptrs[] = {c->d_sub, c->d_pp, c->d_boys, c->d_bscale, c->d_DJ, c->d_DA, c->d_DB, c->d_PM, c->d_AJ,
-                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_off, c->d_kstart};
+                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_off, c->d_ks};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int X = 0; X < kNPairClass; ++X)

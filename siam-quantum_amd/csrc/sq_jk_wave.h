@@ -103,18 +103,21 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
   const int n = A.n;
 
   for (long long t = blockIdx.x; t < A.total; t += gridDim.x) {
-    int ib, ik;
+    int ib, ik, nppb, nppk;
     if constexpr (MODE == kSchwarz) {
       ib = ik = (int)t;
+      nppb = nppk = A.bra[ib].npp;
     } else {
-      ib = find_bra(A.off, A.nbra, t);
-      ik = (int)(t - A.off[ib]) + (A.kstart ? A.kstart[ib] : 0);
-      const PairRec bp = A.bra[ib], kp = A.ket[ik];
-      const double* PM = A.PM;
-      const int nr = A.nref;
-      const double mp = PM[bp.pa * nr + bp.pb] + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
-                        PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
-      if (bp.sch * kp.sch * mp < A.fixed_cut) continue;   // uniform in the workgroup
+      const Segment sg = A.seg[seg_of(A.seg, A.nseg, t)];
+      const long long q = t - sg.t0;
+      if (q >= sg.count) continue;                          // padding slot, uniform
+      const long long* off = A.off + sg.row0;
+      const int r = row_of(off, sg.nrows, q);
+      ib = sg.bra0 + r;
+      ik = sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]);
+      nppb = sg.nppb;
+      nppk = sg.nppk;
+      if (!density_screen_pass(A, A.bra[ib], A.ket[ik])) continue;   // uniform
     }
     const PairRec bp = A.bra[ib], kp = A.ket[ik];
     const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
@@ -123,10 +126,10 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
 #pragma unroll
     for (int k = 0; k < W_::NOUT; ++k) acc[k] = R(0);
 
-    for (int u = 0; u < bp.npp; ++u) {
-      const PrimPair P = A.pp[bp.pp_off + u];
-      for (int v = 0; v < kp.npp; ++v) {
-        const PrimPair Q = A.pp[kp.pp_off + v];
+    for (int u = 0; u < nppb; ++u) {
+      const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
+      for (int v = 0; v < nppk; ++v) {
+        const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
         const double s = P.p + Q.p;
         const double rs = 1.0 / sqrt(s);
         if constexpr (MODE != kSchwarz) {

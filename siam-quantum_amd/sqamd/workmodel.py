@@ -1,13 +1,14 @@
-"""Algorithmic FP64 work per primitive quartet, per angular class.
+"""Algorithmic work per primitive quartet, per angular class.
 
-Counted from the loop structure of the two kernels (FMA = 2 flops,
-multiply/add = 1, division and square root = 1, exp = 1):
-  per-thread kernel (sq_jk_dev.h) for classes with <= 81 integrals per
-  quartet: geometry, Boys (9-term Taylor + downward recursion), Hermite
-  R tensor (level recursion), ket Hermite coefficients, the W transform
-  and the bra contraction, summed over the bra-component slices;
-  cooperative kernel (sq_jk_wave.h) above 81: Boys per order, closed-form
-  R entries, Hermite tables, W and the contraction.
+The count is the one-pass McMurchie-Davidson evaluation that the kernels
+implement (FMA = 2 flops, multiply/add = 1, division and square root = 1,
+exp = 1), independent of how a kernel splits a quartet over lanes:
+geometry and prefactor, Boys (9-term Taylor from the table + downward
+recursion), the Hermite Coulomb tensor R (level recursion), the ket Hermite
+coefficients, W[cd][tuv] = sum E^cd R for every ket component, and the bra
+contraction for every integral.  Work that a kernel repeats (R and Boys per
+ket-component slice, bra Hermite coefficients per bra primitive) is not
+counted, so achieved/peak is a fraction of useful work.
 Digestion is counted per contracted quartet: 12 flops per integral (RHF),
 20 (UHF).  The table is a model; the rocprofv3 FP64 instruction counters of
 the same run are its cross-check.
@@ -53,52 +54,20 @@ def _R_level_flops(L):
     return f
 
 
-def _slices(la, lb, lc, ld):
-    nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
-    sz = max(1, min(36 // ncd, nab)) if ncd <= 36 else 1
-    return [list(range(s, min(s + sz, nab))) for s in range(0, nab, sz)]
-
-
 @lru_cache(maxsize=None)
 def flops_per_prim_quartet(la, lb, lc, ld):
     L = la + lb + lc + ld
     lab = la + lb
-    nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
-    nint = ncart(la) * nb * nc * nd
-    if nint <= 81:
-        total = 0
-        for sl in _slices(la, lb, lc, ld):
-            f = 20 + (28 + 3 * L) + _R_level_flops(L) + 3 * _herm_coef_flops(lc, ld)
-            comps = [(cart(la, ab // nb), cart(lb, ab % nb)) for ab in sl]
-            mx = max(a[0] + b[0] for a, b in comps)
-            my = max(a[1] + b[1] for a, b in comps)
-            mz = max(a[2] + b[2] for a, b in comps)
-            for cd in range(nc * nd):
-                c, d = cart(lc, cd // nd), cart(ld, cd % nd)
-                kx, ky, kz = c[0] + d[0], c[1] + d[1], c[2] + d[2]
-                nw = sum(1 for t in range(mx + 1) for u in range(my + 1) for v in range(mz + 1) if t + u + v <= lab)
-                f += nw * ((kx + 1) * (ky + 1) * (2 * (kz + 1)) + 3 * (kx + 1) * (ky + 1))
-                for a, b in comps:
-                    nx, ny, nz = a[0] + b[0], a[1] + b[1], a[2] + b[2]
-                    f += (nx + 1) * (ny + 1) * (2 * (nz + 1)) + 3 * (nx + 1) * (ny + 1) + 2
-            total += f
-        return total
-    # cooperative kernel
-    f = 20 + (L + 1) * 30
-    f += 3 * _herm_coef_flops(la, lb) + 3 * _herm_coef_flops(lc, ld)
-    for T in range(L + 1):
-        for U in range(L + 1 - T):
-            for V in range(L + 1 - T - U):
-                terms = (T // 2 + 1) * (U // 2 + 1) * (V // 2 + 1)
-                f += terms * 2 + (T + U + V)
+    nab, nb, nc, nd = ncart(la) * ncart(lb), ncart(lb), ncart(lc), ncart(ld)
+    f = 20 + (28 + 3 * L) + _R_level_flops(L) + 3 * _herm_coef_flops(lc, ld)
     for cd in range(nc * nd):
         c, d = cart(lc, cd // nd), cart(ld, cd % nd)
         kx, ky, kz = c[0] + d[0], c[1] + d[1], c[2] + d[2]
         f += nherm(lab) * ((kx + 1) * (ky + 1) * (2 * (kz + 1)) + 3 * (kx + 1) * (ky + 1))
-    for ab in range(ncart(la) * nb):
-        a, b = cart(la, ab // nb), cart(lb, ab % nb)
-        nx, ny, nz = a[0] + b[0], a[1] + b[1], a[2] + b[2]
-        f += nc * nd * ((nx + 1) * (ny + 1) * (2 * (nz + 1)) + 3 * (nx + 1) * (ny + 1) + 2)
+        for ab in range(nab):
+            a, b = cart(la, ab // nb), cart(lb, ab % nb)
+            nx, ny, nz = a[0] + b[0], a[1] + b[1], a[2] + b[2]
+            f += (nx + 1) * (ny + 1) * (2 * (nz + 1)) + 3 * (nx + 1) * (ny + 1) + 2
     return f
 
 
