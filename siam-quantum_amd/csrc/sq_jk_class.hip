@@ -31,7 +31,8 @@ int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
     return 0;
   } else {
     constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
-    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S, R>), dim3(grid), dim3(256), lds, st, a);
+    constexpr int bs = class_block<LA, LB, LC, LD>();
+    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S, R, bs>), dim3(grid), dim3(bs), lds, st, a);
     if (hipGetLastError() != hipSuccess) return -1;
     const int rest = launch_from<MODE, S + 1, R>(a, grid, st);
     return rest < 0 ? -1 : rest + 1;
@@ -44,14 +45,19 @@ int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
+// grid <= 0: size the grid from the work (per-thread kernel: one lane per
+// slot, at most 2^21 lanes; wave kernel: one 64-lane workgroup per slot, at
+// most 2^20)
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
-    // grid counts 256-thread blocks of the per-thread kernel; the wave
-    // kernel uses one 64-thread workgroup per quartet in flight
-    return launch_wave<MODE, R>(a, grid * 4, st);
+    const long long g = grid > 0 ? grid : (a.total < (1 << 20) ? a.total : (1 << 20));
+    return launch_wave<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   } else {
-    return launch_from<MODE, 0, R>(a, grid, st);
+    constexpr int bs = class_block<LA, LB, LC, LD>();
+    const long long b = (a.total + bs - 1) / bs, bmax = (1 << 21) / bs;
+    const long long g = grid > 0 ? grid : (b < bmax ? b : bmax);
+    return launch_from<MODE, 0, R>(a, (int)(g > 0 ? g : 1), st);
   }
 }
 }  // namespace

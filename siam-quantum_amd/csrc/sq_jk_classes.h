@@ -14,8 +14,8 @@
 namespace sqjk {
 // Launches every slice of the class in `mode` (kRHF, kUHF or kSchwarz; the
 // latter only for LA==LC, LB==LD), evaluating integrals in FP32 when fp32 is
-// set (kRHF/kUHF only).  Returns the number of launches, or -1 if a launch
-// failed.
+// set (kRHF/kUHF only).  grid <= 0 sizes the grid from args.total.  Returns
+// the number of launches, or -1 if a launch failed.
 typedef int (*ClassLauncher)(const ClassArgs& a, int mode, int fp32, int grid, hipStream_t st);
 }
 

@@ -36,12 +36,14 @@ struct PairRec {
   double sch;             // Schwarz bound of the parent pair (block maximum)
 };
 
-// primitive pair data, 48 B
+// primitive pair data, 64 B
 struct __attribute__((aligned(16))) PrimPair {
   double p;          // alpha_a + alpha_b
   double Px, Py, Pz;
   double cc;         // c_a c_b exp(-alpha_a alpha_b AB^2 / p) / p
   double eb;         //         exp(-alpha_a alpha_b AB^2 / p) / p
+  double hp;         // 1 / (2p)
+  double pad;
 };
 
 // One bra group x ket group block of a launch.  Row r (bra pair bra0 + r)
@@ -71,10 +73,17 @@ struct ClassArgs {
   const double* __restrict__ DB;      // beta density for K (UHF)
   const double* __restrict__ PM;      // per reference-shell-pair max |P|
   const double* __restrict__ bscale;  // per basis function component scale
-  double* AJ;                         // J half-accumulator (n x n)
-  double* AKA;                        // K half-accumulators
+  double* AJ;                         // J half-accumulator (n x n), wave kernel
+  double* AKA;                        // K half-accumulators, wave kernel
   double* AKB;
   double* sch_out;                    // Schwarz mode output (n x n)
+  // Pair-indexed digestion of the per-thread kernel (lanes hold consecutive
+  // ket pairs, so these atomics are contiguous across a wavefront):
+  double* JpB;                        // [NAB][bra pair] J contributions, bra class
+  double* JpK;                        // [NCD][ket pair] J contributions, ket class
+  double* SA;                         // [n][NC+ND][ket pair] K staging, alpha
+  double* SB;                         // same, beta (UHF)
+  int npb, npk;                       // pairs in the bra / ket class
   long long total;                    // slots (RHF/UHF), pairs (Schwarz)
   double fixed_cut;                   // density-weighted screening cut
   int nseg, same, n, nref;
@@ -148,32 +157,34 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
   R acc[NAB][NS];
   sfor<NAB>([&](auto a) { sfor<NS>([&](auto c) { acc[a][c] = R(0); }); });
 
-  for (int u = 0; u < nppb; ++u) {
-    const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
-    const R h2p = R(0.5 / P.p);
-    Herm1<LA, LB, R> Ex, Ey, Ez;
-    herm_coef<LA, LB, R>(Ex, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
-    herm_coef<LA, LB, R>(Ey, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
-    herm_coef<LA, LB, R>(Ez, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
-    for (int v = 0; v < nppk; ++v) {
-      const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
+  // ket primitive outer (per-lane data, loaded once), bra primitive inner
+  // (the same record for every lane of a bra-uniform wavefront)
+  for (int v = 0; v < nppk; ++v) {
+    const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
+    const R h2q = R(Q.hp);
+    Herm1<LC, LD, R> Fx, Fy, Fz;
+    herm_coef<LC, LD, R>(Fx, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
+    herm_coef<LC, LD, R>(Fy, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
+    herm_coef<LC, LD, R>(Fz, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
+    for (int u = 0; u < nppb; ++u) {
+      const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
       const double s = P.p + Q.p;
-      const double rs = 1.0 / sqrt(s);
+      const double rs = rsqrt_nr(s);
       if constexpr (MODE != kSchwarz) {
         if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;
       }
-      const R alpha = R(P.p * Q.p / s);
+      const R alpha = R(P.p * Q.p * (rs * rs));
       const R X = R(P.Px - Q.Px), Y = R(P.Py - Q.Py), Z = R(P.Pz - Q.Pz);
       const R T = alpha * (X * X + Y * Y + Z * Z);
       R F[L + 1];
       boys<L, R>(T, F, lds);
       R Rt[nherm(L)];
       herm_R<L, R>(Rt, F, alpha, X, Y, Z);
-      const R h2q = R(0.5 / Q.p);
-      Herm1<LC, LD, R> Fx, Fy, Fz;
-      herm_coef<LC, LD, R>(Fx, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
-      herm_coef<LC, LD, R>(Fy, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
-      herm_coef<LC, LD, R>(Fz, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
+      const R h2p = R(P.hp);
+      Herm1<LA, LB, R> Ex, Ey, Ez;
+      herm_coef<LA, LB, R>(Ex, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
+      herm_coef<LA, LB, R>(Ey, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
+      herm_coef<LA, LB, R>(Ez, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
       const R pref = R(kTwoPi25 * P.cc * Q.cc * rs);
 
       sfor<NS>([&](auto csI) {
@@ -293,7 +304,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       sfor<NAB>([&](auto abI) {
         constexpr int ab = decltype(abI)::value;
         const double s = wave_sum(jab[ab]);
-        if (lead) fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * s);
+        if (lead) fatomic(&A.JpB[(size_t)ab * A.npb + ib], 2.0 * s);
       });
     }
 #endif
@@ -301,17 +312,21 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       if (!bra_uniform)
         sfor<NAB>([&](auto abI) {
           constexpr int ab = decltype(abI)::value;
-          fatomic(&A.AJ[(fa + ab / NB) * n + fb + ab % NB], 2.0 * jab[ab]);
+          fatomic(&A.JpB[(size_t)ab * A.npb + ib], 2.0 * jab[ab]);
         });
       sfor<NS>([&](auto csI) {
         constexpr int cd = CD0 + decltype(csI)::value;
-        fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * jcd[csI.value]);
+        fatomic(&A.JpK[(size_t)cd * A.npk + ik], 2.0 * jcd[csI.value]);
       });
     }
 
     // Exchange (per spin): A_K[a,c] += v D[b,d]; A_K[b,c] += v D[a,d];
     //                      A_K[a,d] += v D[b,c]; A_K[b,d] += v D[a,c]
-    auto exchange = [&](const double* __restrict__ D, double* AK) {
+    // staged as S[row][comp][ket pair] with comp = c component, or NC + d
+    // component; a later pass adds S into the rows of A_K.
+    constexpr int NCOMP = NC + ND;
+    const size_t npk = (size_t)A.npk;
+    auto exchange = [&](const double* __restrict__ D, double* S) {
       constexpr int C0 = CD0 / ND, C1 = (CD1 - 1) / ND;  // ket c-range of the slice
       constexpr int NCr = C1 - C0 + 1;
       constexpr int D0 = (NCr > 1) ? 0 : CD0 % ND;
@@ -340,16 +355,18 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       });
       if (!valid) return;
       sfor<NA>([&](auto I) {
-        sfor<NCr>([&](auto J) { fatomic(&AK[(fa + I.value) * n + fc + C0 + J.value], kac[I.value][J.value]); });
-        sfor<NDr>([&](auto J) { fatomic(&AK[(fa + I.value) * n + fd + D0 + J.value], kad[I.value][J.value]); });
+        const size_t row = (size_t)(fa + I.value) * NCOMP;
+        sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kac[I.value][J.value]); });
+        sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kad[I.value][J.value]); });
       });
       sfor<NB>([&](auto I) {
-        sfor<NCr>([&](auto J) { fatomic(&AK[(fb + I.value) * n + fc + C0 + J.value], kbc[I.value][J.value]); });
-        sfor<NDr>([&](auto J) { fatomic(&AK[(fb + I.value) * n + fd + D0 + J.value], kbd[I.value][J.value]); });
+        const size_t row = (size_t)(fb + I.value) * NCOMP;
+        sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kbc[I.value][J.value]); });
+        sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kbd[I.value][J.value]); });
       });
     };
-    exchange(A.DA, A.AKA);
-    if constexpr (MODE == kUHF) exchange(A.DB, A.AKB);
+    exchange(A.DA, A.SA);
+    if constexpr (MODE == kUHF) exchange(A.DB, A.SB);
   }
 }
 
@@ -367,8 +384,10 @@ __host__ __device__ __forceinline__ bool density_screen_pass(const ClassArgs& A,
 // One launch per (class pair, precision, mode, slice S).  RHF/UHF: grid-stride
 // over wavefronts; a wavefront takes 64 consecutive slots of one segment.
 // Schwarz: grid-stride over the pairs of the class, diagonal quartets.
-template <int LA, int LB, int LC, int LD, int MODE, int S, typename R = double>
-__global__ void __launch_bounds__(256) k_class(ClassArgs A) {
+// BS: workgroup size; the LDS Boys table is shared by the BS/64 wavefronts of
+// a workgroup, so classes with few registers per lane use large workgroups.
+template <int LA, int LB, int LC, int LD, int MODE, int S, typename R = double, int BS = 256>
+__global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
   using SI = SliceInfo<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
   constexpr int CD0 = S * SI::SZ;
@@ -405,6 +424,15 @@ __global__ void __launch_bounds__(256) k_class(ClassArgs A) {
       quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, ib, ik, sg.nppb, sg.nppk, valid, lds);
     }
   }
+}
+
+// Workgroup size of the per-thread kernel: 1024 lanes (16 wavefronts share
+// one LDS Boys table, up to 128 VGPRs per lane) for classes holding at most 9
+// integrals per lane, else 256.
+template <int LA, int LB, int LC, int LD>
+constexpr int class_block() {
+  using SI = SliceInfo<LA, LB, LC, LD>;
+  return SI::NAB * SI::SZ <= 9 ? 1024 : 256;
 }
 
 template <int LA, int LB, int LC, int LD>

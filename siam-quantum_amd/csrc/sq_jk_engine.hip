@@ -117,6 +117,58 @@ __global__ void k_finalize(const double* __restrict__ AJ, const double* __restri
   }
 }
 
+// A_J[fa+ia][fb+ib] += Jp[ab][i] for every pair i of one class.  Every
+// (pair, component) owns one element of A_J, so plain read-modify-write is
+// safe (the class kernels have finished).
+__global__ void k_reduce_j(const double* __restrict__ Jp, const PairRec* __restrict__ pr,
+                           const SubShell* __restrict__ sh, int np, int nb, int nab, double* AJ, int n) {
+  const long long tot = (long long)np * nab;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double v = Jp[e];
+    if (v == 0.0) continue;
+    const int ab = (int)(e / np), i = (int)(e - (long long)ab * np);
+    const PairRec p = pr[i];
+    AJ[(size_t)(sh[p.a].bf_off + ab / nb) * n + sh[p.b].bf_off + ab % nb] += v;
+  }
+}
+
+// Exchange staging of the per-thread kernels, one ket class per slot:
+// S[x][comp][j] belongs to row x of A_K at column bf(c of pair j) + comp
+// (comp < nc) or bf(d of pair j) + comp - nc.
+struct KStage {
+  const double* S[6];
+  const PairRec* pr[6];
+  int np[6], nc[6], ncomp[6];
+};
+
+// One workgroup per row x of A_K: the row is summed in LDS, then added to A_K.
+__global__ void __launch_bounds__(256) k_reduce_k(KStage R, const SubShell* __restrict__ sh, double* AK, int n) {
+  extern __shared__ double row[];
+  for (int x = blockIdx.x; x < n; x += gridDim.x) {
+    for (int y = threadIdx.x; y < n; y += blockDim.x) row[y] = 0.0;
+    __syncthreads();
+    for (int Y = 0; Y < 6; ++Y) {
+      if (!R.S[Y]) continue;
+      const int np = R.np[Y], nc = R.nc[Y], ncomp = R.ncomp[Y];
+      const double* Sx = R.S[Y] + (size_t)x * ncomp * np;
+      const PairRec* prY = R.pr[Y];
+      const long long tot = (long long)ncomp * np;
+      for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+        const double v = Sx[e];
+        if (v == 0.0) continue;
+        const int comp = (int)(e / np), j = (int)(e - (long long)comp * np);
+        const PairRec p = prY[j];
+        const int col = comp < nc ? sh[p.a].bf_off + comp : sh[p.b].bf_off + comp - nc;
+        __hip_atomic_fetch_add(&row[col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < n; y += blockDim.x) AK[(size_t)x * n + y] += row[y];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 struct sqjk_ctx {
@@ -153,6 +205,11 @@ struct sqjk_ctx {
   int* d_ks = nullptr;
   size_t seg_cap = 0, off_cap = 0, ks_cap = 0;
   long long plan_q = 0, plan_q32 = 0;
+  // pair-indexed digestion buffers per class: Jp [NAB][np], S [n][NC+ND][np]
+  double* d_Jp[kNPairClass] = {};
+  double* d_SA[kNPairClass] = {};
+  double* d_SB[kNPairClass] = {};
+  size_t jp_len[kNPairClass] = {}, s_len[kNPairClass] = {};
   double mixed_thr = 0.0;  // Schwarz products below this run in FP32
   std::vector<double> h_DJ, h_PM, h_J, h_KA, h_KB;
   sqjk_stats last{};
@@ -264,6 +321,7 @@ void plan_pairs(sqjk_ctx* c, const std::vector<double>& pexp, const std::vector<
           q.Px = (ea * A.x + eb * B.x) / p; q.Py = (ea * A.y + eb * B.y) / p; q.Pz = (ea * A.z + eb * B.z) / p;
           q.eb = K;
           q.cc = pcoef[A.prim_off + ka] * pcoef[B.prim_off + kb] * K;
+          q.hp = 0.5 / p;
           c->pp.push_back(q);
         }
       c->pairs[pair_class(A.l, B.l)].push_back(pr);
@@ -305,6 +363,23 @@ int upload_pairs(sqjk_ctx* c) {
   return SQJK_OK;
 }
 
+// Pair-indexed digestion buffers (alpha always, beta for UHF builds).
+int ensure_stage(sqjk_ctx* c, bool uhf) {
+  for (int X = 0; X < kNPairClass; ++X) {
+    const size_t np = c->pairs[X].size();
+    const int nab = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]);
+    const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
+    c->jp_len[X] = np * nab;
+    c->s_len[X] = (size_t)c->n * ncomp * np;
+    if (np == 0) continue;
+    int rc;
+    if (!c->d_Jp[X] && (rc = dalloc(&c->d_Jp[X], c->jp_len[X]))) return rc;
+    if (!c->d_SA[X] && (rc = dalloc(&c->d_SA[X], c->s_len[X]))) return rc;
+    if (uhf && !c->d_SB[X] && (rc = dalloc(&c->d_SB[X], c->s_len[X]))) return rc;
+  }
+  return SQJK_OK;
+}
+
 ClassArgs base_args(sqjk_ctx* c) {
   ClassArgs a{};
   a.sh = c->d_sub; a.pp = c->d_pp; a.boys = c->d_boys; a.bscale = c->d_bscale;
@@ -326,7 +401,7 @@ int compute_schwarz(sqjk_ctx* c) {
     a.total = (long long)c->pairs[X].size();
     a.same = 1;
     ClassLauncher f = launcher_for(kPairL[X][0], kPairL[X][1], kPairL[X][0], kPairL[X][1]);
-    if (!f || f(a, kSchwarz, 0, grid_for(a.total, 256), c->st) < 0)
+    if (!f || f(a, kSchwarz, 0, 0, c->st) < 0)
       return fail(SQJK_EHIP, "Schwarz kernel launch failed");
   }
   c->sch_basis.resize(nn);
@@ -551,12 +626,22 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   const double thr64 = std::max(thr, c->mixed_thr);
   int rc = make_plan(c, thr, thr64);
   if (rc) return rc;
+  if ((size_t)n * sizeof(double) > 64 * 1024) return fail(SQJK_EARG, "nBasis above 8192 is not supported");
+  if ((rc = ensure_stage(c, uhf))) return rc;
+  HIPCHK(hipEventRecord(c->ev0, c->st));
   const int fill_grid = grid_for((long long)nn, 256);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, nn, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AKA, nn, 0.0);
   if (uhf) hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AKB, nn, 0.0);
+  for (int X = 0; X < kNPairClass; ++X) {
+    if (c->pairs[X].empty()) continue;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)c->jp_len[X], 256)), dim3(256), 0, c->st, c->d_Jp[X],
+                       c->jp_len[X], 0.0);
+    const int sg = std::max(1, (int)std::min<size_t>((c->s_len[X] + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SA[X], c->s_len[X], 0.0);
+    if (uhf) hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SB[X], c->s_len[X], 0.0);
+  }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev0, c->st));
   int launches = 0;
   for (const sqjk_ctx::Launch& L : c->launches) {
     ClassArgs a = base_args(c);
@@ -569,11 +654,35 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     a.same = (L.X == L.Y);
     a.total = L.slots;
     a.fixed_cut = fixedCutoff;
+    a.JpB = c->d_Jp[L.X];
+    a.JpK = c->d_Jp[L.Y];
+    a.SA = c->d_SA[L.Y];
+    a.SB = c->d_SB[L.Y];
+    a.npb = (int)c->pairs[L.X].size();
+    a.npk = (int)c->pairs[L.Y].size();
     ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
-    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, grid_for(L.slots, 256), c->st) : -1;
+    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, 0, c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
   }
+  // fold the pair-indexed buffers into the half-accumulators
+  KStage RA{}, RB{};
+  for (int X = 0; X < kNPairClass; ++X) {
+    const int np = (int)c->pairs[X].size();
+    if (np == 0) continue;
+    const int nb = sqmd::ncart(kPairL[X][1]), nab = sqmd::ncart(kPairL[X][0]) * nb;
+    hipLaunchKernelGGL(k_reduce_j, dim3(grid_for((long long)np * nab, 256)), dim3(256), 0, c->st, c->d_Jp[X],
+                       c->d_pairs[X], c->d_sub, np, nb, nab, c->d_AJ, n);
+    RA.S[X] = c->d_SA[X];
+    RB.S[X] = c->d_SB[X];
+    RA.pr[X] = RB.pr[X] = c->d_pairs[X];
+    RA.np[X] = RB.np[X] = np;
+    RA.nc[X] = RB.nc[X] = sqmd::ncart(kPairL[X][0]);
+    RA.ncomp[X] = RB.ncomp[X] = sqmd::ncart(kPairL[X][0]) + nb;
+  }
+  hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(256), (size_t)n * sizeof(double), c->st, RA, c->d_sub, c->d_AKA, n);
+  if (uhf) hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(256), (size_t)n * sizeof(double), c->st, RB, c->d_sub, c->d_AKB, n);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev1, c->st));
   hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, c->d_J,
                      c->d_KA, c->d_KB, n, uhf);
@@ -655,8 +764,12 @@ void sqjk_destroy(sqjk_ctx* c) {
                   c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_off, c->d_ks};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (int X = 0; X < kNPairClass; ++X)
+  for (int X = 0; X < kNPairClass; ++X) {
     if (c->d_pairs[X]) (void)hipFree(c->d_pairs[X]);
+    if (c->d_Jp[X]) (void)hipFree(c->d_Jp[X]);
+    if (c->d_SA[X]) (void)hipFree(c->d_SA[X]);
+    if (c->d_SB[X]) (void)hipFree(c->d_SB[X]);
+  }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);

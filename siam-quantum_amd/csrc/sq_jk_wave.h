@@ -131,11 +131,11 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       for (int v = 0; v < nppk; ++v) {
         const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
         const double s = P.p + Q.p;
-        const double rs = 1.0 / sqrt(s);
+        const double rs = rsqrt_nr(s);
         if constexpr (MODE != kSchwarz) {
           if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;   // uniform
         }
-        const double alpha_d = P.p * Q.p / s;
+        const double alpha_d = P.p * Q.p * (rs * rs);
         const double Xd = P.Px - Q.Px, Yd = P.Py - Q.Py, Zd = P.Pz - Q.Pz;
         const double T = alpha_d * (Xd * Xd + Yd * Yd + Zd * Zd);
         const R X = R(Xd), Y = R(Yd), Z = R(Zd);
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
           for (int m = 0; m < lane; ++m) pw *= -alpha_d;
           g[lane] = R(pw * boys_one(lane, T, A.boys));
         }
-        const R h2p = R(0.5 / P.p), h2q = R(0.5 / Q.p);
+        const R h2p = R(P.hp), h2q = R(Q.hp);
         if (lane == 60) herm_axis_to_lds<LA, LB, R>(Ea + 0 * W_::EA, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
         if (lane == 61) herm_axis_to_lds<LA, LB, R>(Ea + 1 * W_::EA, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
         if (lane == 62) herm_axis_to_lds<LA, LB, R>(Ea + 2 * W_::EA, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);

@@ -81,6 +81,20 @@ __device__ __forceinline__ void boys_load_lds(double* lds, const double* __restr
   }
 }
 
+// 1/sqrt(s) for s > 0: the hardware estimate refined by two Newton steps
+// y <- y (3/2 - s/2 y^2), each of which squares the relative error.
+__host__ __device__ __forceinline__ double rsqrt_nr(double s) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double y = __builtin_amdgcn_rsq(s);
+  const double h = 0.5 * s;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+#else
+  return 1.0 / sqrt(s);
+#endif
+}
+
 __host__ __device__ __forceinline__ double rexp(double x) { return exp(x); }
 __host__ __device__ __forceinline__ float rexp(float x) { return expf(x); }
 __host__ __device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
