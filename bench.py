@@ -42,6 +42,48 @@ def make_inputs(name):
     return mol, bas, A + A.T
 
 
+def scf_solves(name, nsolve, mixed_thr):
+    """Repeated RHF SCF solves through the uhf() drop-in (reference defaults:
+    3-step cut-offs, DIIS4, SCFConv 1e-8): each solve builds its per-molecule
+    state, runs the SCF on the device and releases the state, as uhf() does.
+    Returns FP64 and mixed-precision solves/s with p50/p99 latency and the
+    energy difference."""
+    import ctypes as C
+    from sqamd import inputs, lib, default_option, dptr, GTOBasis, Molecule, Option
+    nmol, seed = WORKLOADS[name]
+    Z, X = inputs.water_cluster(nmol, seed)
+    mol = inputs.mol_from_angstrom(Z, X)
+    bas = inputs.make_basis(mol, "631g_star")
+    L = lib()
+    L.uhf.argtypes = [C.c_int, C.POINTER(GTOBasis), C.POINTER(Molecule), C.c_int, C.c_int] + \
+        [C.POINTER(C.c_double)] * 4 + [C.POINTER(Option)]
+    L.uhf.restype = C.c_double
+    L.sqjk_set_dropin_mixed.argtypes = [C.c_double]
+    n, ne = bas.n, mol.n_electrons
+    out = {"workload": f"(H2O){nmol} RHF/6-31G* SCF, {n} basis functions, seed {seed}, "
+                       f"reference defaults with SCFConv 1e-8", "solves": nsolve}
+    for label, thr in (("fp64", 0.0), ("mixed", mixed_thr)):
+        if label == "mixed" and thr <= 0:
+            continue
+        L.sqjk_set_dropin_mixed(thr)
+        lat, E = [], None
+        for _ in range(nsolve):
+            CA, CB, eA, eB = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+            opt = default_option(SCFConv=1e-8)
+            t0 = time.perf_counter()
+            E = L.uhf(n, bas.ptr, mol.ptr, ne // 2, ne // 2, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt))
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        out[label] = {"value": len(lat) / lat.sum(), "unit": "SCF solves/s",
+                      "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                      "energy": E}
+        if label == "mixed":
+            out[label]["threshold"] = thr
+            out[label]["abs_dE"] = abs(E - out["fp64"]["energy"])
+    L.sqjk_set_dropin_mixed(0.0)
+    return out
+
+
 def cpu_baseline(bas, P, cut, slices):
     """Reference code (oracle/_ref) on the PQ % slices == 0 part of the same
     build, one core; builds/s scaled by the number of slices."""
@@ -76,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-threshold", type=float, default=1e-4,
                     help="Schwarz product below which quartets run in FP32 for the mixed line (0: skip)")
+    ap.add_argument("--scf-solves", type=int, default=5, help="SCF solves per precision (0: skip)")
+    ap.add_argument("--scf-workload", default="w16", choices=sorted(WORKLOADS))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -158,6 +202,8 @@ def main():
                         "max_abs_dJ": float(np.abs(Jm - Jd).max()), "max_abs_dK": float(np.abs(Km - Kd).max()),
                         "speedup_vs_fp64": (a.steps / wall_m) / out["value"]}
     eng.close()
+    if a.scf_solves > 0:
+        out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(bas, P, a.cutoff, a.cpu_slices)

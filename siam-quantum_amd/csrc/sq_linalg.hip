@@ -1,7 +1,10 @@
 // sq_linalg.hip - dense FP64 linear algebra of the SCF on the device.
 //
-// Generalised symmetric eigenproblem F C = S C e with rocSOLVER (dsygvd),
-// behind the reference's src/lin.h surface:
+// Generalised symmetric eigenproblem F C = S C e on the device, behind the
+// reference's src/lin.h surface.  S is fixed during an SCF, so its Cholesky
+// factor S = U^T U and X = U^-1 are computed once per metric (rocSOLVER
+// dpotrf/dtrtri) and every solve is X^T F X (two DGEMMs), rocSOLVER dsyevd
+// and C = X C' (one DGEMM):
 //   gen_sym_eigen      <- src/lin.h:20 (src/lin.c:66-177, LAPACK dspgvd)
 //   gen_sym_SomeEigen  <- src/lin.h:24 (src/lin.c:195-307, LAPACK dspgvx)
 //   linear_solver      <- src/lin.h:30 (src/lin.c:320-391, LAPACK dgesv)
@@ -11,6 +14,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,8 +29,10 @@ namespace {
 struct EigWork {
   int n = 0;
   rocblas_handle h = nullptr;
-  double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr, *P = nullptr;
+  double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr, *P = nullptr, *X = nullptr;
   rocblas_int* info = nullptr;
+  int metric_n = 0;             // X = U^-1 is valid for this n and the host copy below
+  std::vector<double> metric;   // S of the cached factor
 };
 std::mutex g_mu;
 EigWork g_w;
@@ -36,17 +42,52 @@ int ensure(int n) {
   if (g_w.n >= n) return 0;
   if (g_w.A) {
     (void)hipFree(g_w.A); (void)hipFree(g_w.B); (void)hipFree(g_w.W); (void)hipFree(g_w.E);
-    (void)hipFree(g_w.P); (void)hipFree(g_w.info);
+    (void)hipFree(g_w.P); (void)hipFree(g_w.X); (void)hipFree(g_w.info);
   }
+  g_w.metric_n = 0;
   const size_t nn = (size_t)n * n;
   if (hipMalloc(&g_w.A, nn * 8) != hipSuccess || hipMalloc(&g_w.B, nn * 8) != hipSuccess ||
-      hipMalloc(&g_w.P, nn * 8) != hipSuccess ||
+      hipMalloc(&g_w.P, nn * 8) != hipSuccess || hipMalloc(&g_w.X, nn * 8) != hipSuccess ||
       hipMalloc(&g_w.W, (size_t)n * 8) != hipSuccess || hipMalloc(&g_w.E, (size_t)n * 8) != hipSuccess ||
       hipMalloc(&g_w.info, sizeof(rocblas_int)) != hipSuccess) {
     g_w.n = 0;
     return -1;
   }
   g_w.n = n;
+  return 0;
+}
+
+// strictly lower triangle (column-major) set to zero
+__global__ void k_zero_lower(double* X, int n) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    if (i > j) X[e] = 0.0;
+  }
+}
+
+int read_info() {
+  rocblas_int info = 0;
+  if (hipMemcpy(&info, g_w.info, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  return info != 0 ? -5 : 0;
+}
+
+// X = U^-1 with S = U^T U (upper Cholesky factor), unless S is the cached metric
+int set_metric(int n, const double* S) {
+  const size_t nn = (size_t)n * n;
+  if (g_w.metric_n == n && std::memcmp(g_w.metric.data(), S, nn * sizeof(double)) == 0) return 0;
+  g_w.metric_n = 0;
+  if (hipMemcpy(g_w.X, S, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
+  if (rocsolver_dpotrf(g_w.h, rocblas_fill_upper, n, g_w.X, n, g_w.info) != rocblas_status_success) return -4;
+  if (int rc = read_info()) return rc;
+  if (rocsolver_dtrtri(g_w.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, g_w.X, n, g_w.info) !=
+      rocblas_status_success)
+    return -4;
+  if (int rc = read_info()) return rc;
+  hipLaunchKernelGGL(k_zero_lower, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, 0, g_w.X, n);
+  if (hipGetLastError() != hipSuccess) return -3;
+  g_w.metric.assign(S, S + nn);
+  g_w.metric_n = n;
   return 0;
 }
 
@@ -58,18 +99,25 @@ int sqla_gen_eigen(int n, const double* F, const double* S, double* e, double* C
   if (n < 1) return -1;
   std::lock_guard<std::mutex> lk(g_mu);
   if (ensure(n)) return -2;
+  if (int rc = set_metric(n, S)) return rc;
   const size_t nn = (size_t)n * n;
-  // symmetric input: row-major == column-major
+  const double one = 1.0, zero = 0.0;
+  // symmetric input: row-major == column-major.  A' = X^T F X
   if (hipMemcpy(g_w.A, F, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  if (hipMemcpy(g_w.B, S, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  rocblas_status st = rocsolver_dsygvd(g_w.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
-                                       g_w.A, n, g_w.B, n, g_w.W, g_w.E, g_w.info);
-  if (st != rocblas_status_success) return -4;
-  rocblas_int info = 0;
-  if (hipMemcpy(&info, g_w.info, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (info != 0) return -5;
-  // column k of A (column-major) is orbital k: A[k*n + i]
-  if (hipMemcpy(C, g_w.A, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.A, n, g_w.X, n,
+                    &zero, g_w.P, n) != rocblas_status_success ||
+      rocblas_dgemm(g_w.h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
+                    &zero, g_w.A, n) != rocblas_status_success)
+    return -4;
+  if (rocsolver_dsyevd(g_w.h, rocblas_evect_original, rocblas_fill_upper, n, g_w.A, n, g_w.W, g_w.E, g_w.info) !=
+      rocblas_status_success)
+    return -4;
+  if (int rc = read_info()) return rc;
+  // C = X C'; column k (column-major) is orbital k: C[k*n + i]
+  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.A, n,
+                    &zero, g_w.P, n) != rocblas_status_success)
+    return -4;
+  if (hipMemcpy(C, g_w.P, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   if (hipMemcpy(e, g_w.W, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   return 0;
 }

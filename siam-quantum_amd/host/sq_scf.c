@@ -16,8 +16,11 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <pthread.h>
 #include <stdlib.h>
+#include <unistd.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../include/sq_jk.h"
 #include "../../include/sq_scf.h"
@@ -167,16 +170,66 @@ static double total_energy(int n, const struct Molecule_t *mol, const struct opt
 	return E;
 }
 
+/* One-electron matrices H = T + V (+ field) and S, lower triangle rows split
+ * over host threads (rows are claimed from a shared counter; the integral
+ * routines keep their scratch in thread-local storage). */
+struct one_e_job {
+	int n;
+	const struct GTOBasis_t *gto;
+	const struct Molecule_t *mol;
+	double *H, *S;
+	int next;
+	pthread_mutex_t lock;
+};
+
+static void *one_e_worker(void *arg)
+{
+	struct one_e_job *jb = (struct one_e_job *)arg;
+	const int n = jb->n;
+	for (;;) {
+		pthread_mutex_lock(&jb->lock);
+		const int i = jb->next++;
+		pthread_mutex_unlock(&jb->lock);
+		if (i >= n) break;
+		for (int j = 0; j <= i; j++) {
+			const double h = GTO_kinetic(i, j, jb->gto) + GTO_nuclei(i, j, jb->gto, jb->mol);
+			jb->H[i * n + j] = jb->H[j * n + i] = h;
+			const double s = GTO_overlap(i, j, jb->gto);
+			jb->S[i * n + j] = jb->S[j * n + i] = s;
+		}
+	}
+	return NULL;
+}
+
+static double now_s(void)
+{
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+/* host threads: OMP_NUM_THREADS if set, else the online cores, at most 64 */
+static int host_threads(void)
+{
+	const char *env = getenv("OMP_NUM_THREADS");
+	long t = env ? strtol(env, NULL, 10) : sysconf(_SC_NPROCESSORS_ONLN);
+	if (t < 1) t = 1;
+	if (t > 64) t = 64;
+	return (int)t;
+}
+
 static void one_electron(int n, const struct GTOBasis_t *gto, const struct Molecule_t *mol,
                          const struct option_t *opt, double *H, double *S)
 {
-	for (int i = 0; i < n; i++)
-		for (int j = 0; j <= i; j++) {
-			const double h = GTO_kinetic(i, j, gto) + GTO_nuclei(i, j, gto, mol);
-			H[i * n + j] = H[j * n + i] = h;
-			const double s = GTO_overlap(i, j, gto);
-			S[i * n + j] = S[j * n + i] = s;
-		}
+	struct one_e_job jb = {n, gto, mol, H, S, 0, PTHREAD_MUTEX_INITIALIZER};
+	int nt = host_threads();
+	if (nt > n) nt = n;
+	pthread_t th[64];
+	int started = 0;
+	for (int t = 1; t < nt; t++)
+		if (pthread_create(&th[started], NULL, one_e_worker, &jb) == 0) started++;
+	one_e_worker(&jb);
+	for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
 	if (opt->Ex != 0.0 || opt->Ey != 0.0 || opt->Ez != 0.0)
 		for (int i = 0; i < n; i++)
 			for (int j = 0; j < n; j++)
@@ -217,7 +270,11 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 	ALLOC(JT, nn); ALLOC(dJT, nn); ALLOC(KA, nn); ALLOC(dKA, nn); ALLOC(KB, nn); ALLOC(dKB, nn);
 	ALLOC(PA, nn); ALLOC(dPA, nn); ALLOC(PB, nn); ALLOC(dPB, nn); ALLOC(FA, nn); ALLOC(FB, nn);
 	ALLOC(H, nn); ALLOC(S, nn);
+	/* SQ_SCF_TIMING=1 prints the wall time of each phase to stderr */
+	const int timing = getenv("SQ_SCF_TIMING") != NULL;
+	double tp[7] = {0}, t0 = now_s();
 	one_electron(n, gto, mol, opt, H, S);
+	tp[0] += now_s() - t0;
 
 	switch (opt->SCFGuess) {
 	case SCFGUESS_CACB:
@@ -254,7 +311,9 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 #define GETCUTOFF(acc) ((acc) / (sA * sA + sA * sB + sB * sB) / 50.0)
 	const double realCutoff = GETCUTOFF(opt->SCFConv);
 	opt->SCFCutoff = realCutoff;
+	t0 = now_s();
 	double *Schwarz = create_Schwarz(n, gto);
+	tp[1] += now_s() - t0;
 
 	conv_diis4(0, 0.0, NULL, NULL);
 	const double gamma = opt->SCFDrag;
@@ -277,7 +336,10 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 	int iter = 0, notConverged = 1;
 	do {
 		iter++;
+		t0 = now_s();
 		uhf_getGMatrix(n, gto, Schwarz, thisCutoff, dPA, opt->restricted ? dPA : dPB, dJT, dKA, dKB, opt);
+		tp[2] += now_s() - t0;
+		t0 = now_s();
 		for (size_t e = 0; e < nn; e++) {
 			JT[e] += dJT[e];
 			KA[e] += dKA[e];
@@ -287,12 +349,26 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			FA[e] = H[e] + JT[e] + KA[e];
 			FB[e] = H[e] + JT[e] + KB[e];
 		}
+		tp[5] += now_s() - t0;
 		/* the device eigen-solver returns S-orthonormal orbitals, so the
-		 * reference's re-normalisation (src/uhf.c:1705-1706) is a no-op here */
+		 * reference's re-normalisation (src/uhf.c:1705-1706) is a no-op here.
+		 * Restricted: KB = KA, so FB = FA and the beta solve would repeat
+		 * the alpha one exactly. */
+		t0 = now_s();
 		eigen_or_die(n, nEA, FA, S, eA, CA);
-		eigen_or_die(n, nEB, FB, S, eB, CB);
+		if (opt->restricted && nEA == nEB) {
+			memcpy(CB, CA, nn * sizeof(double));
+			memcpy(eB, eA, (size_t)n * sizeof(double));
+		} else {
+			eigen_or_die(n, nEB, FB, S, eB, CB);
+		}
+		tp[3] += now_s() - t0;
+		t0 = now_s();
 		density_or_die(n, nEA, CA, PA);
-		density_or_die(n, nEB, CB, PB);
+		if (opt->restricted && nEA == nEB) memcpy(PB, PA, nn * sizeof(double));
+		else density_or_die(n, nEB, CB, PB);
+		tp[4] += now_s() - t0;
+		t0 = now_s();
 		dE = Etot;
 		Etot = total_energy(n, mol, opt, PA, PB, H, JT, KA, KB);
 		dE = Etot - dE;
@@ -302,6 +378,7 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			dPA[e] = PA[e] - dPA[e];
 			dPB[e] = PB[e] - dPB[e];
 		}
+		tp[5] += now_s() - t0;
 		if (iter >= opt->SCFMax) break;
 		if (opt->SCFAccuracy == SCFACCURACY_3STEP) {
 			if (!notConverged) break;
@@ -318,8 +395,13 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 	/* release the J/K state (the reference resets its statics here) */
 	{
 		double z = 0.0;
+		t0 = now_s();
 		GTO_JK_Matrix_Quartet_Parallel(0, 0, &z, &z, gto, &z, 0.0, &z, &z, &z, opt);
+		tp[6] += now_s() - t0;
 	}
+	if (timing)
+		fprintf(stderr, "uhf timing [s]: one-electron %.3f  Schwarz %.3f  J/K %.3f  eigen %.3f  density %.3f"
+		        "  host %.3f  release %.3f  (%d iterations)\n", tp[0], tp[1], tp[2], tp[3], tp[4], tp[5], tp[6], iter);
 	conv_diis4(0, 0.0, NULL, NULL);
 	free(Schwarz);
 	free(JT); free(dJT); free(KA); free(dKA); free(KB); free(dKB);
