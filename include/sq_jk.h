@@ -64,8 +64,10 @@ int sqjk_build(sqjk_ctx *ctx, const double *PA, const double *PB, double fixedCu
                double scfCutoff, int restricted, double *JT, double *KA, double *KB);
 
 /* Mixed precision: shell quartets whose Schwarz product Sch_PQ*Sch_IJ is
- * below `threshold` are evaluated in FP32 (Boys, Hermite and contraction);
- * larger ones stay FP64, and all digestion and accumulation is FP64.
+ * below `threshold` are evaluated in FP32 (Boys, Hermite and contraction)
+ * in the angular classes of at most 81 integrals per quartet; larger ones
+ * and all other classes stay FP64, and all digestion and accumulation is
+ * FP64.
  * threshold 0 (the default) keeps every quartet in FP64.  No counterpart in
  * the reference (north-star mixed-precision mode). */
 int sqjk_set_mixed(sqjk_ctx *ctx, double threshold);
@@ -82,6 +84,15 @@ int sqjk_last_stats(const sqjk_ctx *ctx, sqjk_stats *st);
  * primitive cut-off), prim_quartets[k] = their primitive quartets.  Any
  * output pointer may be NULL.  Returns the number of classes. */
 int sqjk_class_stats(const sqjk_ctx *ctx, int *lq, long long *quartets, long long *prim_quartets);
+
+/* Per-class kernel timing.  With `on` set, every later sqjk_build records a
+ * pair of HIP events around each class launch group on the context's stream
+ * (all ket-component slices, FP64 and FP32 launches of one class) and
+ * sqjk_class_times returns their durations in ms, in the class order of
+ * sqjk_class_stats.  Off by default; no counterpart in the reference
+ * (measurement support). */
+int sqjk_set_profile(sqjk_ctx *ctx, int on);
+int sqjk_class_times(const sqjk_ctx *ctx, double *kernel_ms);
 
 /* Number of basis functions, reference shells and pure-l sub-shells. */
 int sqjk_dims(const sqjk_ctx *ctx, int *nBasis, int *nShell, int *nSubShell);

@@ -20,10 +20,14 @@ using namespace sqjk;
 constexpr int LA = SQ_LA, LB = SQ_LB, LC = SQ_LC, LD = SQ_LD;
 using SI = SliceInfo<LA, LB, LC, LD>;
 
-// Classes up to 324 integrals per quartet use the per-thread kernel (one
-// launch per ket-component slice); (dd|dp) and (dd|dd) the cooperative
+// Classes up to 648 integrals per quartet use the per-thread kernel (one
+// launch per ket-component slice); (dd|dd) the cooperative
 // one-wavefront-per-quartet kernel.
-constexpr bool kWave = SI::NAB * SI::NCD > 324;
+constexpr bool kWave = SI::NAB * SI::NCD > 648;
+// FP32 evaluation is compiled for classes of at most 81 integrals; larger
+// classes evaluate their FP32 range in FP64 (FP32 and FP64 vector rates are
+// equal on this target without packed FP32, so this costs no time).
+constexpr bool kFP32 = SI::NAB * SI::NCD <= 81;
 
 template <int MODE, int S, typename R>
 int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
@@ -65,9 +69,13 @@ int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
 int SQ_LAUNCHER(const sqjk::ClassArgs& a, int mode, int fp32, int grid, hipStream_t st) {
   switch (mode) {
     case sqjk::kRHF:
-      return fp32 ? launch_mode<sqjk::kRHF, float>(a, grid, st) : launch_mode<sqjk::kRHF, double>(a, grid, st);
+      if constexpr (kFP32)
+        if (fp32) return launch_mode<sqjk::kRHF, float>(a, grid, st);
+      return launch_mode<sqjk::kRHF, double>(a, grid, st);
     case sqjk::kUHF:
-      return fp32 ? launch_mode<sqjk::kUHF, float>(a, grid, st) : launch_mode<sqjk::kUHF, double>(a, grid, st);
+      if constexpr (kFP32)
+        if (fp32) return launch_mode<sqjk::kUHF, float>(a, grid, st);
+      return launch_mode<sqjk::kUHF, double>(a, grid, st);
 #if SQ_LA == SQ_LC && SQ_LB == SQ_LD
     case sqjk::kSchwarz: return fp32 ? -1 : launch_mode<sqjk::kSchwarz, double>(a, grid, st);
 #endif

@@ -206,6 +206,9 @@ struct sqjk_ctx {
   size_t seg_cap = 0, off_cap = 0, ks_cap = 0;
   long long plan_q = 0, plan_q32 = 0;
   // pair-indexed digestion buffers per class: Jp [NAB][np], S [n][NC+ND][np]
+  bool profile = false;
+  std::vector<hipEvent_t> prof_ev;                      // 2 per launch
+  double cls_ms[kNPairClass][kNPairClass] = {};
   double* d_Jp[kNPairClass] = {};
   double* d_SA[kNPairClass] = {};
   double* d_SB[kNPairClass] = {};
@@ -467,7 +470,12 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y)
       for (int fp = 0; fp < 2; ++fp) {
-        if (fp == 1 && !(thr64 > thr)) continue;
+        // FP32 evaluation exists for classes of at most 81 integrals per
+        // quartet; larger classes keep their whole range in FP64
+        const int nint = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]) * sqmd::ncart(kPairL[Y][0]) *
+                         sqmd::ncart(kPairL[Y][1]);
+        const double thr64c = nint <= 81 ? thr64 : thr;
+        if (fp == 1 && !(thr64c > thr)) continue;
         const auto &bra = c->pairs[X], &ket = c->pairs[Y];
         sqjk_ctx::Launch L{X, Y, fp, (int)c->h_seg.size(), 0, 0, 0};
         for (size_t a = 0; a < c->groups[X].size(); ++a)
@@ -483,7 +491,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
             for (int i = 0; i < g1.count; ++i) {
               const double si = bra[g1.begin + i].sch;
               while (lo > 0 && !(si * ket[g2.begin + lo - 1].sch >= thr)) --lo;
-              while (lo64 > 0 && !(si * ket[g2.begin + lo64 - 1].sch >= thr64)) --lo64;
+              while (lo64 > 0 && !(si * ket[g2.begin + lo64 - 1].sch >= thr64c)) --lo64;
               int cnt = lo, c64 = std::min(lo64, lo);
               if (X == Y && a == b) { cnt = std::min(cnt, i + 1); c64 = std::min(c64, i + 1); }
               if (cnt == 0) break;
@@ -643,7 +651,14 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   }
   HIPCHK(hipGetLastError());
   int launches = 0;
-  for (const sqjk_ctx::Launch& L : c->launches) {
+  if (c->profile && c->prof_ev.size() < 2 * c->launches.size()) {
+    const size_t have = c->prof_ev.size();
+    c->prof_ev.resize(2 * c->launches.size());
+    for (size_t e = have; e < c->prof_ev.size(); ++e) HIPCHK(hipEventCreate(&c->prof_ev[e]));
+  }
+  for (size_t li = 0; li < c->launches.size(); ++li) {
+    const sqjk_ctx::Launch& L = c->launches[li];
+    if (c->profile) HIPCHK(hipEventRecord(c->prof_ev[2 * li], c->st));
     ClassArgs a = base_args(c);
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
@@ -664,6 +679,7 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, 0, c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
+    if (c->profile) HIPCHK(hipEventRecord(c->prof_ev[2 * li + 1], c->st));
   }
   // fold the pair-indexed buffers into the half-accumulators
   KStage RA{}, RB{};
@@ -707,6 +723,13 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  std::memset(c->cls_ms, 0, sizeof(c->cls_ms));
+  if (c->profile)
+    for (size_t li = 0; li < c->launches.size(); ++li) {
+      float lm = 0.f;
+      HIPCHK(hipEventElapsedTime(&lm, c->prof_ev[2 * li], c->prof_ev[2 * li + 1]));
+      c->cls_ms[c->launches[li].X][c->launches[li].Y] += lm;
+    }
   c->last.kernel_ms = ms;
   c->last.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->last.quartets = c->plan_q;
@@ -748,6 +771,20 @@ int sqjk_class_stats(const sqjk_ctx* c, int* lq, long long* quartets, long long*
   return k;
 }
 
+int sqjk_set_profile(sqjk_ctx* c, int on) {
+  if (!c) return fail(SQJK_EARG, "sqjk_set_profile: null context");
+  c->profile = on != 0;
+  return SQJK_OK;
+}
+
+int sqjk_class_times(const sqjk_ctx* c, double* kernel_ms) {
+  if (!c || !kernel_ms) return fail(SQJK_EARG, "sqjk_class_times: null argument");
+  int k = 0;
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++k) kernel_ms[k] = c->cls_ms[X][Y];
+  return k;
+}
+
 int sqjk_dims(const sqjk_ctx* c, int* nBasis, int* nShell, int* nSubShell) {
   if (!c) return fail(SQJK_EARG, "sqjk_dims: null context");
   if (nBasis) *nBasis = c->n;
@@ -770,6 +807,7 @@ void sqjk_destroy(sqjk_ctx* c) {
     if (c->d_SA[X]) (void)hipFree(c->d_SA[X]);
     if (c->d_SB[X]) (void)hipFree(c->d_SB[X]);
   }
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);

@@ -38,6 +38,8 @@ def lib():
     L.sqjk_last_stats.argtypes = [C.c_void_p, C.POINTER(SqjkStats)]
     L.sqjk_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.sqjk_destroy.argtypes = [C.c_void_p]
+    L.sqjk_set_profile.argtypes = [C.c_void_p, C.c_int]
+    L.sqjk_class_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.sqjk_class_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)]
     L.sqjk_errmsg.restype = C.c_char_p
@@ -99,6 +101,15 @@ class JKEngine:
         p = (C.c_longlong * 21)()
         k = lib().sqjk_class_stats(self._h, lq, q, p)
         return [((lq[4 * i], lq[4 * i + 1], lq[4 * i + 2], lq[4 * i + 3]), q[i], p[i]) for i in range(k)]
+
+    def set_profile(self, on=True):
+        _check(lib().sqjk_set_profile(self._h, int(on)), "sqjk_set_profile")
+
+    def class_times(self):
+        """Kernel ms per class of the last build (profile mode), class_stats order."""
+        t = (C.c_double * 21)()
+        k = lib().sqjk_class_times(self._h, t)
+        return [t[i] for i in range(k)]
 
     def dims(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
