@@ -29,17 +29,16 @@ constexpr bool kWave = SI::NAB * SI::NCD > 648;
 // equal on this target without packed FP32, so this costs no time).
 constexpr bool kFP32 = SI::NAB * SI::NCD <= 81;
 
-template <int MODE, int S, typename R>
-int launch_from(const ClassArgs& a, int grid, hipStream_t st) {
-  if constexpr (S >= SI::NSLICE || kWave) {
+// all slices in one launch: grid (gx, NSLICE)
+template <int MODE, typename R>
+int launch_thread(const ClassArgs& a, int gx, hipStream_t st) {
+  if constexpr (kWave) {
     return 0;
   } else {
     constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
     constexpr int bs = class_block<LA, LB, LC, LD>();
-    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, S, R, bs>), dim3(grid), dim3(bs), lds, st, a);
-    if (hipGetLastError() != hipSuccess) return -1;
-    const int rest = launch_from<MODE, S + 1, R>(a, grid, st);
-    return rest < 0 ? -1 : rest + 1;
+    hipLaunchKernelGGL((k_class<LA, LB, LC, LD, MODE, R, bs>), dim3(gx, SI::NSLICE), dim3(bs), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
 
@@ -50,8 +49,8 @@ int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
 }
 
 // grid <= 0: size the grid from the work (per-thread kernel: one lane per
-// slot, at most 2^21 lanes; wave kernel: one 64-lane workgroup per slot, at
-// most 2^20)
+// slot and slice, at most 2^21 lanes per slice; wave kernel: one 64-lane
+// workgroup per slot, at most 2^20)
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
@@ -61,7 +60,7 @@ int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
     constexpr int bs = class_block<LA, LB, LC, LD>();
     const long long b = (a.total + bs - 1) / bs, bmax = (1 << 21) / bs;
     const long long g = grid > 0 ? grid : (b < bmax ? b : bmax);
-    return launch_from<MODE, 0, R>(a, (int)(g > 0 ? g : 1), st);
+    return launch_thread<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   }
 }
 }  // namespace

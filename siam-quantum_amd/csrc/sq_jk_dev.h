@@ -381,49 +381,57 @@ __host__ __device__ __forceinline__ bool density_screen_pass(const ClassArgs& A,
   return !(bp.sch * kp.sch * mp < A.fixed_cut);
 }
 
-// One launch per (class pair, precision, mode, slice S).  RHF/UHF: grid-stride
-// over wavefronts; a wavefront takes 64 consecutive slots of one segment.
-// Schwarz: grid-stride over the pairs of the class, diagonal quartets.
+// One launch per (class pair, precision, mode); blockIdx.y selects the
+// ket-component slice (a compile-time branch per slice, uniform in the
+// workgroup), so the slices of a class fill the device together.
+// RHF/UHF: grid-stride over wavefronts; a wavefront takes 64 consecutive
+// slots of one segment.  Schwarz: grid-stride over the pairs of the class,
+// diagonal quartets.
 // BS: workgroup size; the LDS Boys table is shared by the BS/64 wavefronts of
 // a workgroup, so classes with few registers per lane use large workgroups.
-template <int LA, int LB, int LC, int LD, int MODE, int S, typename R = double, int BS = 256>
+template <int LA, int LB, int LC, int LD, int MODE, typename R = double, int BS = 256>
 __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
   using SI = SliceInfo<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
-  constexpr int CD0 = S * SI::SZ;
-  constexpr int CD1 = (CD0 + SI::SZ < SI::NCD) ? CD0 + SI::SZ : SI::NCD;
-  static_assert(CD0 < CD1, "empty ket-component slice");
   extern __shared__ double lds[];
   boys_load_lds<L>(lds, A.boys);
   __syncthreads();
-  if constexpr (MODE == kSchwarz) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
-      const int npp = A.bra[t].npp;
-      quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, npp, true, lds);
+  const int slice = blockIdx.y;
+  sfor<SI::NSLICE>([&](auto sI) {
+    constexpr int S = decltype(sI)::value;
+    constexpr int CD0 = S * SI::SZ;
+    constexpr int CD1 = (CD0 + SI::SZ < SI::NCD) ? CD0 + SI::SZ : SI::NCD;
+    static_assert(CD0 < CD1, "empty ket-component slice");
+    if (S != slice) return;
+    if constexpr (MODE == kSchwarz) {
+      const long long stride = (long long)gridDim.x * blockDim.x;
+      for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
+        const int npp = A.bra[t].npp;
+        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, npp, true, lds);
+      }
+    } else {
+      const int lane = threadIdx.x & 63;
+      const int wpb = blockDim.x >> 6;
+      const long long nw = A.total >> 6;
+      const long long wstride = (long long)gridDim.x * wpb;
+      const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+      for (long long w = (long long)blockIdx.x * wpb + wib; w < nw; w += wstride) {
+        const long long t0 = w << 6;
+        const Segment sg = A.seg[seg_of(A.seg, A.nseg, t0)];
+        const long long* off = A.off + sg.row0;
+        const long long q0 = t0 - sg.t0;  // < sg.count: segments start on 64-slot boundaries
+        int r = row_of(off, sg.nrows, q0);
+        const long long q = q0 + lane;
+        const bool in = q < sg.count;
+        if (in)
+          while (off[r + 1] <= q) ++r;
+        const int ib = sg.bra0 + r;
+        const int ik = in ? sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]) : sg.ket0;
+        const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
+        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, ib, ik, sg.nppb, sg.nppk, valid, lds);
+      }
     }
-  } else {
-    const int lane = threadIdx.x & 63;
-    const int wpb = blockDim.x >> 6;
-    const long long nw = A.total >> 6;
-    const long long wstride = (long long)gridDim.x * wpb;
-    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (long long w = (long long)blockIdx.x * wpb + wib; w < nw; w += wstride) {
-      const long long t0 = w << 6;
-      const Segment sg = A.seg[seg_of(A.seg, A.nseg, t0)];
-      const long long* off = A.off + sg.row0;
-      const long long q0 = t0 - sg.t0;  // < sg.count: segments start on 64-slot boundaries
-      int r = row_of(off, sg.nrows, q0);
-      const long long q = q0 + lane;
-      const bool in = q < sg.count;
-      if (in)
-        while (off[r + 1] <= q) ++r;
-      const int ib = sg.bra0 + r;
-      const int ik = in ? sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]) : sg.ket0;
-      const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
-      quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, ib, ik, sg.nppb, sg.nppk, valid, lds);
-    }
-  }
+  });
 }
 
 // Workgroup size of the per-thread kernel: 1024 lanes (16 wavefronts share
