@@ -48,17 +48,20 @@ int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
-// grid <= 0: size the grid from the work (per-thread kernel: one lane per
-// slot and slice, at most 2^21 lanes per slice; wave kernel: one 64-lane
-// workgroup per slot, at most 2^20)
+// grid <= 0: size the grid from the work, a.total = tiles (RHF/UHF) or pairs
+// (Schwarz).  Per-thread kernel: one wavefront per tile (one lane per pair)
+// and slice, at most 2^21 lanes per slice; wave kernel: one 64-lane
+// workgroup per tile or pair, at most 2^20.
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
-    const long long g = grid > 0 ? grid : (a.total < (1 << 20) ? a.total : (1 << 20));
+    const long long items = MODE == kSchwarz ? a.total : a.total * 64;
+    const long long g = grid > 0 ? grid : (items < (1 << 20) ? items : (1 << 20));
     return launch_wave<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   } else {
     constexpr int bs = class_block<LA, LB, LC, LD>();
-    const long long b = (a.total + bs - 1) / bs, bmax = (1 << 21) / bs;
+    const long long lanes = MODE == kSchwarz ? a.total : a.total * 64;
+    const long long b = (lanes + bs - 1) / bs, bmax = (1 << 21) / bs;
     const long long g = grid > 0 ? grid : (b < bmax ? b : bmax);
     return launch_thread<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   }

@@ -8,10 +8,11 @@
 //
 // Work lists are wavefront-uniform: a launch covers one class pair and one
 // precision, and is cut into segments (bra group x ket group, where a group
-// holds the pairs of one class with the same number of primitive pairs).
-// Every segment starts on a 64-slot boundary, so all lanes of a wavefront run
-// the same primitive loop counts; consecutive lanes take consecutive ket pairs
-// of one bra pair (rows of Schwarz-sorted ket prefixes).
+// holds the pairs of one class with the same number of primitive pairs), so
+// all lanes of a wavefront run the same primitive loop counts.  A tile is a
+// run of bra rows with the same first shell a and a chunk of 64 consecutive
+// ket pairs: lane l keeps ket pair k0 + l for every row of the run, so its J
+// and a-side exchange contributions stay in registers across the run.
 #pragma once
 #include "sq_md.h"
 
@@ -46,15 +47,18 @@ struct __attribute__((aligned(16))) PrimPair {
   double pad;
 };
 
-// One bra group x ket group block of a launch.  Row r (bra pair bra0 + r)
-// owns the quartets [off[row0 + r], off[row0 + r + 1]) of the segment; its
-// ket pairs are ket0 + kstart[row0 + r] + (0, 1, ...).
+// One bra group x ket group block of a launch.
 struct Segment {
-  long long t0;      // first slot of the segment in the launch (multiple of 64)
-  long long count;   // quartets in the segment
-  int bra0, nrows;   // first bra pair (class-list index), rows
-  int ket0, row0;    // first ket pair of the ket group, first off[]/kstart[] entry
+  int ket0;          // class-list index of the first ket pair of the ket group
   int nppb, nppk;    // primitive pairs of every bra / ket pair of the segment
+  int pad;
+};
+
+// Row r: bra pair rbra[r] (class-list index) with the ket pairs at positions
+// [rlo[r], rhi[r]) of its segment's ket group.  A tile covers rows [r0, r1)
+// (one bra shell a) and ket positions [k0, k0 + 64).
+struct Tile {
+  int seg, r0, r1, k0;
 };
 
 enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
@@ -65,8 +69,10 @@ struct ClassArgs {
   const PairRec* __restrict__ ket;
   const PrimPair* __restrict__ pp;
   const Segment* __restrict__ seg;    // segments of this launch
-  const long long* __restrict__ off;  // per segment: nrows+1 offsets from row0
-  const int* __restrict__ kstart;     // per row: first ket within the group (NULL: 0)
+  const Tile* __restrict__ tile;      // tiles of this launch
+  const int* __restrict__ rbra;       // rows: bra pair
+  const int* __restrict__ rlo;        // rows: first ket position
+  const int* __restrict__ rhi;        // rows: end of the ket positions
   const double* __restrict__ boys;    // Boys table (global)
   const double* __restrict__ DJ;      // density for J (PA+PB)
   const double* __restrict__ DA;      // alpha density for K
@@ -84,9 +90,9 @@ struct ClassArgs {
   double* SA;                         // [n][NC+ND][ket pair] K staging, alpha
   double* SB;                         // same, beta (UHF)
   int npb, npk;                       // pairs in the bra / ket class
-  long long total;                    // slots (RHF/UHF), pairs (Schwarz)
+  long long total;                    // tiles (RHF/UHF), pairs (Schwarz)
   double fixed_cut;                   // density-weighted screening cut
-  int nseg, same, n, nref;
+  int same, n, nref;
 };
 
 using namespace sqmd;
@@ -111,26 +117,6 @@ __host__ __device__ __forceinline__ void fatomic(double* p, double v) {
 #endif
 }
 
-// last segment whose first slot is <= t
-__host__ __device__ __forceinline__ int seg_of(const Segment* __restrict__ seg, int nseg, long long t) {
-  int lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (seg[mid].t0 <= t) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// row r with off[r] <= q < off[r+1], for 0 <= q < off[nrows]
-__host__ __device__ __forceinline__ int row_of(const long long* __restrict__ off, int nrows, long long q) {
-  int lo = 0, hi = nrows;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= q) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 // Sum over the 64 lanes of the wavefront (all lanes active).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -138,14 +124,47 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Ket c- and d-component ranges covered by the ket-component slice [CD0, CD1).
+template <int ND, int CD0, int CD1>
+struct KetRange {
+  static constexpr int C0 = CD0 / ND, C1 = (CD1 - 1) / ND;
+  static constexpr int NCr = C1 - C0 + 1;
+  static constexpr int D0 = (NCr > 1) ? 0 : CD0 % ND;
+  static constexpr int D1 = (NCr > 1) ? ND - 1 : (CD1 - 1) % ND;
+  static constexpr int NDr = D1 - D0 + 1;
+};
+
+// Contributions a lane keeps in registers over a run of bra rows with the
+// same shell a and the same ket pair: J_cd and the a-side exchange blocks
+// (A_K[a,c], A_K[a,d]) per spin.
+template <int LA, int LC, int LD, int CD0, int CD1>
+struct RunAcc {
+  static constexpr int NA = ncart(LA), ND = ncart(LD), NS = CD1 - CD0;
+  using KR = KetRange<ND, CD0, CD1>;
+  double jcd[NS];
+  double kac[2][NA][KR::NCr];
+  double kad[2][NA][KR::NDr];
+  __host__ __device__ __forceinline__ void zero() {
+    sfor<NS>([&](auto I) { jcd[I.value] = 0.0; });
+    sfor<2>([&](auto S) {
+      sfor<NA>([&](auto I) {
+        sfor<KR::NCr>([&](auto J) { kac[S.value][I.value][J.value] = 0.0; });
+        sfor<KR::NDr>([&](auto J) { kad[S.value][I.value][J.value] = 0.0; });
+      });
+    });
+  }
+};
+
 // Contracted quartet (ib, ik), ket components [CD0, CD1).  R: precision of
 // the integral evaluation (double, or float for the mixed-precision range);
 // digestion and accumulation are FP64.  All lanes of a wavefront call this
 // together; `valid` gates the lane's contributions, so cross-lane reductions
 // see every lane.
-template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE, typename R = double>
+// RUN: J_cd and the a-side exchange go to `run` instead of memory.
+template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE, typename R = double, bool RUN = false>
 __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, int nppb, int nppk,
-                                                 bool valid, const double* lds) {
+                                                 bool valid, const double* lds,
+                                                 RunAcc<LA, LC, LD, CD0, CD1>* run = nullptr) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int LAB = LA + LB, L = LAB + LC + LD;
   constexpr int NAB = NA * NB, NS = CD1 - CD0;
@@ -308,12 +327,14 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       });
     }
 #endif
-    if (valid) {
-      if (!bra_uniform)
-        sfor<NAB>([&](auto abI) {
-          constexpr int ab = decltype(abI)::value;
-          fatomic(&A.JpB[(size_t)ab * A.npb + ib], 2.0 * jab[ab]);
-        });
+    if (valid && !bra_uniform)
+      sfor<NAB>([&](auto abI) {
+        constexpr int ab = decltype(abI)::value;
+        fatomic(&A.JpB[(size_t)ab * A.npb + ib], 2.0 * jab[ab]);
+      });
+    if constexpr (RUN) {
+      sfor<NS>([&](auto csI) { run->jcd[csI.value] += jcd[csI.value]; });  // zero when !valid
+    } else if (valid) {
       sfor<NS>([&](auto csI) {
         constexpr int cd = CD0 + decltype(csI)::value;
         fatomic(&A.JpK[(size_t)cd * A.npk + ik], 2.0 * jcd[csI.value]);
@@ -326,12 +347,9 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     // component; a later pass adds S into the rows of A_K.
     constexpr int NCOMP = NC + ND;
     const size_t npk = (size_t)A.npk;
-    auto exchange = [&](const double* __restrict__ D, double* S) {
-      constexpr int C0 = CD0 / ND, C1 = (CD1 - 1) / ND;  // ket c-range of the slice
-      constexpr int NCr = C1 - C0 + 1;
-      constexpr int D0 = (NCr > 1) ? 0 : CD0 % ND;
-      constexpr int D1 = (NCr > 1) ? ND - 1 : (CD1 - 1) % ND;
-      constexpr int NDr = D1 - D0 + 1;
+    auto exchange = [&](const double* __restrict__ D, double* S, auto spinI) {
+      using KR = KetRange<ND, CD0, CD1>;
+      constexpr int C0 = KR::C0, NCr = KR::NCr, D0 = KR::D0, NDr = KR::NDr;
       double kac[NA][NCr], kad[NA][NDr], kbc[NB][NCr], kbd[NB][NDr];
       sfor<NA>([&](auto I) { sfor<NCr>([&](auto J) { kac[I.value][J.value] = 0.0; });
                              sfor<NDr>([&](auto J) { kad[I.value][J.value] = 0.0; }); });
@@ -353,21 +371,55 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
           kbd[jb][id - D0] += v * D[ga * n + gc];
         });
       });
+      if constexpr (RUN) {
+        constexpr int sp = decltype(spinI)::value;
+        sfor<NA>([&](auto I) {
+          sfor<NCr>([&](auto J) { run->kac[sp][I.value][J.value] += kac[I.value][J.value]; });
+          sfor<NDr>([&](auto J) { run->kad[sp][I.value][J.value] += kad[I.value][J.value]; });
+        });
+      }
       if (!valid) return;
-      sfor<NA>([&](auto I) {
-        const size_t row = (size_t)(fa + I.value) * NCOMP;
-        sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kac[I.value][J.value]); });
-        sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kad[I.value][J.value]); });
-      });
+      if constexpr (!RUN) {
+        sfor<NA>([&](auto I) {
+          const size_t row = (size_t)(fa + I.value) * NCOMP;
+          sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kac[I.value][J.value]); });
+          sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kad[I.value][J.value]); });
+        });
+      }
       sfor<NB>([&](auto I) {
         const size_t row = (size_t)(fb + I.value) * NCOMP;
         sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kbc[I.value][J.value]); });
         sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kbd[I.value][J.value]); });
       });
     };
-    exchange(A.DA, A.SA);
-    if constexpr (MODE == kUHF) exchange(A.DB, A.SB);
+    exchange(A.DA, A.SA, ic<0>{});
+    if constexpr (MODE == kUHF) exchange(A.DB, A.SB, ic<1>{});
   }
+}
+
+// Writes a run's register contributions (ket pair ik, bra shell starting at
+// basis function fa) to the staging buffers.
+template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE>
+__host__ __device__ __forceinline__ void run_flush(const ClassArgs& A, const RunAcc<LA, LC, LD, CD0, CD1>& run,
+                                                   int fa, int ik) {
+  constexpr int NA = ncart(LA), NC = ncart(LC), ND = ncart(LD), NCOMP = NC + ND;
+  using KR = KetRange<ND, CD0, CD1>;
+  const size_t npk = (size_t)A.npk;
+  sfor<CD1 - CD0>([&](auto csI) {
+    constexpr int cd = CD0 + decltype(csI)::value;
+    fatomic(&A.JpK[(size_t)cd * npk + ik], 2.0 * run.jcd[csI.value]);
+  });
+  sfor<(MODE == kUHF) ? 2 : 1>([&](auto spI) {
+    constexpr int sp = decltype(spI)::value;
+    double* S = sp ? A.SB : A.SA;
+    sfor<NA>([&](auto I) {
+      const size_t row = (size_t)(fa + I.value) * NCOMP;
+      sfor<KR::NCr>([&](auto J) { fatomic(&S[(row + KR::C0 + J.value) * npk + ik], run.kac[sp][I.value][J.value]); });
+      sfor<KR::NDr>([&](auto J) {
+        fatomic(&S[(row + NC + KR::D0 + J.value) * npk + ik], run.kad[sp][I.value][J.value]);
+      });
+    });
+  });
 }
 
 // Density-weighted screening at the reference-shell level (the reference's
@@ -412,23 +464,25 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
     } else {
       const int lane = threadIdx.x & 63;
       const int wpb = blockDim.x >> 6;
-      const long long nw = A.total >> 6;
       const long long wstride = (long long)gridDim.x * wpb;
       const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-      for (long long w = (long long)blockIdx.x * wpb + wib; w < nw; w += wstride) {
-        const long long t0 = w << 6;
-        const Segment sg = A.seg[seg_of(A.seg, A.nseg, t0)];
-        const long long* off = A.off + sg.row0;
-        const long long q0 = t0 - sg.t0;  // < sg.count: segments start on 64-slot boundaries
-        int r = row_of(off, sg.nrows, q0);
-        const long long q = q0 + lane;
-        const bool in = q < sg.count;
-        if (in)
-          while (off[r + 1] <= q) ++r;
-        const int ib = sg.bra0 + r;
-        const int ik = in ? sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]) : sg.ket0;
-        const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
-        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, ib, ik, sg.nppb, sg.nppk, valid, lds);
+      for (long long w = (long long)blockIdx.x * wpb + wib; w < A.total; w += wstride) {
+        const Tile tl = A.tile[w];
+        const Segment sg = A.seg[tl.seg];
+        const int j = tl.k0 + lane;  // ket position in the group
+        RunAcc<LA, LC, LD, CD0, CD1> run;
+        run.zero();
+        bool any = false;
+        for (int r = tl.r0; r < tl.r1; ++r) {
+          const int ib = A.rbra[r];
+          const int lo = A.rlo[r];
+          const bool in = j >= lo && j < A.rhi[r];
+          const int ik = sg.ket0 + (in ? j : lo);  // idle lanes evaluate a valid pair, weight 0
+          any = any || in;
+          const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
+          quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, sg.nppk, valid, lds, &run);
+        }
+        if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, A.sh[A.bra[A.rbra[tl.r0]].a].bf_off, sg.ket0 + j);
       }
     }
   });

@@ -193,17 +193,17 @@ struct sqjk_ctx {
   struct Group { int begin, count, npp; };
   std::vector<Group> groups[kNPairClass];
   // work plan for the cut-offs (thr, thr64) of the last build
-  struct Launch { int X, Y, fp32, seg0, nseg; long long slots, quartets; };
+  struct Launch { int X, Y, fp32, seg0, tile0; long long ntile, quartets; };
   bool plan_ok = false;
   double plan_thr = 0.0, plan_thr64 = 0.0;
   std::vector<Launch> launches;
   std::vector<Segment> h_seg;
-  std::vector<long long> h_off;
-  std::vector<int> h_ks;
+  std::vector<Tile> h_tile;
+  std::vector<int> h_rbra, h_rlo, h_rhi;
   Segment* d_seg = nullptr;
-  long long* d_off = nullptr;
-  int* d_ks = nullptr;
-  size_t seg_cap = 0, off_cap = 0, ks_cap = 0;
+  Tile* d_tile = nullptr;
+  int *d_rbra = nullptr, *d_rlo = nullptr, *d_rhi = nullptr;
+  size_t seg_cap = 0, tile_cap = 0, row_cap = 0;
   long long plan_q = 0, plan_q32 = 0;
   // pair-indexed digestion buffers per class: Jp [NAB][np], S [n][NC+ND][np]
   bool profile = false;
@@ -454,19 +454,36 @@ void finish_pairs(sqjk_ctx* c) {
   c->plan_ok = false;
 }
 
-// Rows and segments of every launch for the cut-offs (thr, thr64): quartets
-// with Schwarz product >= thr64 run in FP64, those in [thr, thr64) in FP32.
-// Host only; make_plan uploads the result.
+// Rows and tiles of every launch for the cut-offs (thr, thr64): quartets with
+// Schwarz product >= thr64 run in FP64, those in [thr, thr64) in FP32.
+// Bra rows are taken in (first shell, Schwarz) order; a tile is up to
+// kTileRows rows of one first shell that meet one chunk of 64 ket positions.
+// Host only; make_plan uploads the result.  SQJK_TILE_ROWS (1-256) sets the
+// rows per tile.  Default 1: on (H2O)32/6-31G* one row per tile measured
+// 22.8 ms per build, 4 rows 23.5 ms, 32 rows 40.2 ms (longer serial tiles cost
+// more than the register-held run contributions save).
+int tile_rows() {
+  static const int v = [] {
+    const char* e = std::getenv("SQJK_TILE_ROWS");
+    const int r = e ? std::atoi(e) : 1;
+    return r < 1 ? 1 : (r > 256 ? 256 : r);
+  }();
+  return v;
+}
+
 void build_plan(sqjk_ctx* c, double thr, double thr64) {
   c->launches.clear();
   c->h_seg.clear();
-  c->h_off.clear();
-  c->h_ks.clear();
+  c->h_tile.clear();
+  c->h_rbra.clear();
+  c->h_rlo.clear();
+  c->h_rhi.clear();
   std::memset(c->cls_q, 0, sizeof(c->cls_q));
   std::memset(c->cls_p, 0, sizeof(c->cls_p));
   c->plan_q = c->plan_q32 = 0;
-  std::vector<long long> offs;
-  std::vector<int> kss;
+  struct Row { int i, lo, hi; };
+  std::vector<Row> rows, sel;
+  std::vector<int> order;
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y)
       for (int fp = 0; fp < 2; ++fp) {
@@ -477,51 +494,70 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
         const double thr64c = nint <= 81 ? thr64 : thr;
         if (fp == 1 && !(thr64c > thr)) continue;
         const auto &bra = c->pairs[X], &ket = c->pairs[Y];
-        sqjk_ctx::Launch L{X, Y, fp, (int)c->h_seg.size(), 0, 0, 0};
+        sqjk_ctx::Launch L{X, Y, fp, (int)c->h_seg.size(), (int)c->h_tile.size(), 0, 0};
         for (size_t a = 0; a < c->groups[X].size(); ++a)
           for (size_t b = 0; b < c->groups[Y].size(); ++b) {
             if (X == Y && b > a) continue;  // canonical: ket pair index <= bra pair index
             const auto &g1 = c->groups[X][a], &g2 = c->groups[Y][b];
-            offs.clear();
-            kss.clear();
+            // kets of the group passing with bra Schwarz si: a prefix (sorted by sch)
+            auto count = [&](double si, double t) {
+              int lo = 0, hi = g2.count;
+              while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (si * ket[g2.begin + mid].sch >= t) lo = mid + 1; else hi = mid;
+              }
+              return lo;
+            };
+            order.resize(g1.count);
+            for (int k = 0; k < g1.count; ++k) order[k] = g1.begin + k;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bra[x].a < bra[y].a; });
+            rows.clear();
             long long acc = 0;
-            // bra and ket groups are sorted by sch, so the ket prefix that
-            // passes can only shrink as the bra index grows: one sweep
-            int lo = g2.count, lo64 = g2.count;
-            for (int i = 0; i < g1.count; ++i) {
-              const double si = bra[g1.begin + i].sch;
-              while (lo > 0 && !(si * ket[g2.begin + lo - 1].sch >= thr)) --lo;
-              while (lo64 > 0 && !(si * ket[g2.begin + lo64 - 1].sch >= thr64c)) --lo64;
-              int cnt = lo, c64 = std::min(lo64, lo);
-              if (X == Y && a == b) { cnt = std::min(cnt, i + 1); c64 = std::min(c64, i + 1); }
-              if (cnt == 0) break;
-              const int k0 = fp ? c64 : 0, k1 = fp ? cnt : c64;
-              offs.push_back(acc);
-              kss.push_back(k0);
-              acc += k1 - k0;
+            for (int i : order) {
+              int cnt = count(bra[i].sch, thr), c64 = std::min(count(bra[i].sch, thr64c), cnt);
+              if (X == Y && a == b) {
+                const int lim = i - g1.begin + 1;
+                cnt = std::min(cnt, lim);
+                c64 = std::min(c64, lim);
+              }
+              const int lo = fp ? c64 : 0, hi = fp ? cnt : c64;
+              if (hi <= lo) continue;
+              rows.push_back({i, lo, hi});
+              acc += hi - lo;
             }
-            while (!offs.empty() && offs.back() == acc) { offs.pop_back(); kss.pop_back(); }
             if (acc == 0) continue;
-            Segment sg{};
-            sg.t0 = L.slots;
-            sg.count = acc;
-            sg.bra0 = g1.begin;
-            sg.nrows = (int)offs.size();
-            sg.ket0 = g2.begin;
-            sg.row0 = (int)c->h_off.size();
-            sg.nppb = g1.npp;
-            sg.nppk = g2.npp;
-            c->h_seg.push_back(sg);
-            c->h_off.insert(c->h_off.end(), offs.begin(), offs.end());
-            c->h_off.push_back(acc);
-            c->h_ks.insert(c->h_ks.end(), kss.begin(), kss.end());
-            c->h_ks.push_back(0);
-            L.slots += (acc + 63) / 64 * 64;
+            const int seg = (int)c->h_seg.size();
+            c->h_seg.push_back({g2.begin, g1.npp, g2.npp, 0});
+            for (size_t r = 0; r < rows.size();) {
+              size_t e = r;  // run of rows with the same first shell
+              int mlo = rows[r].lo, mhi = rows[r].hi;
+              while (e < rows.size() && bra[rows[e].i].a == bra[rows[r].i].a) {
+                mlo = std::min(mlo, rows[e].lo);
+                mhi = std::max(mhi, rows[e].hi);
+                ++e;
+              }
+              for (int k0 = mlo / 64 * 64; k0 < mhi; k0 += 64) {
+                sel.clear();
+                for (size_t q = r; q < e; ++q)
+                  if (rows[q].lo < k0 + 64 && rows[q].hi > k0) sel.push_back(rows[q]);
+                const size_t tr = (size_t)tile_rows();
+                for (size_t q = 0; q < sel.size(); q += tr) {
+                  const int r0 = (int)c->h_rbra.size();
+                  for (size_t u = q; u < std::min(sel.size(), q + tr); ++u) {
+                    c->h_rbra.push_back(sel[u].i);
+                    c->h_rlo.push_back(sel[u].lo);
+                    c->h_rhi.push_back(sel[u].hi);
+                  }
+                  c->h_tile.push_back({seg, r0, (int)c->h_rbra.size(), k0});
+                }
+              }
+              r = e;
+            }
             L.quartets += acc;
-            L.nseg++;
             c->cls_p[X][Y] += acc * g1.npp * g2.npp;
           }
-        if (L.nseg == 0) continue;
+        L.ntile = (long long)c->h_tile.size() - L.tile0;
+        if (L.ntile == 0) continue;
         c->launches.push_back(L);
         c->cls_q[X][Y] += L.quartets;
         c->plan_q += L.quartets;
@@ -532,7 +568,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
 int make_plan(sqjk_ctx* c, double thr, double thr64) {
   if (c->plan_ok && c->plan_thr == thr && c->plan_thr64 == thr64) return SQJK_OK;
   build_plan(c, thr, thr64);
-  if (c->h_off.size() > (size_t)INT32_MAX) return fail(SQJK_ENOMEM, "work plan too large");
+  if (c->h_rbra.size() > (size_t)INT32_MAX) return fail(SQJK_ENOMEM, "work plan too large");
   auto grow = [&](auto** d, size_t& cap, size_t need) -> int {
     if (cap >= need && *d) return SQJK_OK;
     if (*d) { (void)hipFree(*d); *d = nullptr; }
@@ -540,13 +576,25 @@ int make_plan(sqjk_ctx* c, double thr, double thr64) {
     return dalloc(d, cap);
   };
   int rc;
+  const size_t nrow = c->h_rbra.size();
   if ((rc = grow(&c->d_seg, c->seg_cap, c->h_seg.size()))) return rc;
-  if ((rc = grow(&c->d_off, c->off_cap, c->h_off.size()))) return rc;
-  if ((rc = grow(&c->d_ks, c->ks_cap, c->h_ks.size()))) return rc;
-  if (!c->h_seg.empty()) {
+  if ((rc = grow(&c->d_tile, c->tile_cap, c->h_tile.size()))) return rc;
+  if (c->row_cap < nrow || !c->d_rbra) {
+    int** rowbufs[] = {&c->d_rbra, &c->d_rlo, &c->d_rhi};
+    for (int** d : rowbufs) {
+      if (*d) (void)hipFree(*d);
+      *d = nullptr;
+    }
+    c->row_cap = std::max<size_t>(nrow, 1024) * 3 / 2;
+    for (int** d : rowbufs)
+      if ((rc = dalloc(d, c->row_cap))) return rc;
+  }
+  if (!c->h_tile.empty()) {
     HIPCHK(hipMemcpyAsync(c->d_seg, c->h_seg.data(), c->h_seg.size() * sizeof(Segment), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->d_off, c->h_off.data(), c->h_off.size() * sizeof(long long), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->d_ks, c->h_ks.data(), c->h_ks.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_tile, c->h_tile.data(), c->h_tile.size() * sizeof(Tile), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_rbra, c->h_rbra.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_rlo, c->h_rlo.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_rhi, c->h_rhi.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
   }
   c->plan_ok = true;
   c->plan_thr = thr;
@@ -662,12 +710,13 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     ClassArgs a = base_args(c);
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
-    a.seg = c->d_seg + L.seg0;
-    a.nseg = L.nseg;
-    a.off = c->d_off;
-    a.kstart = L.fp32 ? c->d_ks : nullptr;
+    a.seg = c->d_seg;
+    a.tile = c->d_tile + L.tile0;
+    a.rbra = c->d_rbra;
+    a.rlo = c->d_rlo;
+    a.rhi = c->d_rhi;
     a.same = (L.X == L.Y);
-    a.total = L.slots;
+    a.total = L.ntile;
     a.fixed_cut = fixedCutoff;
     a.JpB = c->d_Jp[L.X];
     a.JpK = c->d_Jp[L.Y];
@@ -798,7 +847,7 @@ void sqjk_destroy(sqjk_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   void* ptrs[] = {c->d_sub, c->d_pp, c->d_boys, c->d_bscale, c->d_DJ, c->d_DA, c->d_DB, c->d_PM, c->d_AJ,
-                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_off, c->d_ks};
+                  c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_tile, c->d_rbra, c->d_rlo, c->d_rhi};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int X = 0; X < kNPairClass; ++X) {

@@ -102,21 +102,27 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
   const int lane = threadIdx.x;
   const int n = A.n;
 
-  for (long long t = blockIdx.x; t < A.total; t += gridDim.x) {
-    int ib, ik, nppb, nppk;
+  // RHF/UHF: grid-stride over (tile, ket position) items, the rows of the
+  // tile one after the other; Schwarz: grid-stride over the pairs.
+  const long long items = MODE == kSchwarz ? A.total : A.total * 64;
+  for (long long t = blockIdx.x; t < items; t += gridDim.x) {
+    int r0 = 0, r1 = 1, j = 0, nppb = 0, nppk = 0, ket0 = 0;
+    if constexpr (MODE == kSchwarz) {
+      nppb = nppk = A.bra[t].npp;
+    } else {
+      const Tile tl = A.tile[t >> 6];
+      const Segment sg = A.seg[tl.seg];
+      r0 = tl.r0; r1 = tl.r1; j = tl.k0 + (int)(t & 63);
+      nppb = sg.nppb; nppk = sg.nppk; ket0 = sg.ket0;
+    }
+    for (int r = r0; r < r1; ++r) {
+    int ib, ik;
     if constexpr (MODE == kSchwarz) {
       ib = ik = (int)t;
-      nppb = nppk = A.bra[ib].npp;
     } else {
-      const Segment sg = A.seg[seg_of(A.seg, A.nseg, t)];
-      const long long q = t - sg.t0;
-      if (q >= sg.count) continue;                          // padding slot, uniform
-      const long long* off = A.off + sg.row0;
-      const int r = row_of(off, sg.nrows, q);
-      ib = sg.bra0 + r;
-      ik = sg.ket0 + (A.kstart ? A.kstart[sg.row0 + r] : 0) + (int)(q - off[r]);
-      nppb = sg.nppb;
-      nppk = sg.nppk;
+      if (j < A.rlo[r] || j >= A.rhi[r]) continue;                  // uniform
+      ib = A.rbra[r];
+      ik = ket0 + j;
       if (!density_screen_pass(A, A.bra[ib], A.ket[ik])) continue;   // uniform
     }
     const PairRec bp = A.bra[ib], kp = A.ket[ik];
@@ -315,6 +321,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       }
       wave_sync();
     }
+    }  // rows
   }
 }
 
