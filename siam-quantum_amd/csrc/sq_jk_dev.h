@@ -499,7 +499,7 @@ constexpr int class_block() {
 
 template <int LA, int LB, int LC, int LD>
 constexpr size_t class_lds_bytes() {
-  return sizeof(double) * kBoysNT * (LA + LB + LC + LD + kBoysTaylor);
+  return sizeof(double) * kBoysNT * kBoysLdsW;
 }
 
 }  // namespace sqjk

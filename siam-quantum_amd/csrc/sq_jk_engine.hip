@@ -54,11 +54,12 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 // ------------------------------------------------------------- Boys table
-// F_m(k*step) for m < kBoysNM: series for the top order in long double
+// F_m(k*step) for m < kBoysNM and exp(-k*step) in column kBoysNM: series for
+// the top order in long double
 // (F_m(T) = e^-T sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1))), then the
 // stable downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1).
 std::vector<double> make_boys_table() {
-  std::vector<double> tab((size_t)kBoysNT * kBoysNM);
+  std::vector<double> tab((size_t)kBoysNT * kBoysCols);
   for (int k = 0; k < kBoysNT; ++k) {
     const long double T = (long double)k * (long double)kBoysStep;
     const int top = kBoysNM - 1 + 20;  // start a few orders above, recur down
@@ -73,7 +74,8 @@ std::vector<double> make_boys_table() {
     std::vector<long double> F(top + 1);
     F[top] = f;
     for (int m = top - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + ex) / (2 * m + 1);
-    for (int m = 0; m < kBoysNM; ++m) tab[(size_t)k * kBoysNM + m] = (double)F[m];
+    for (int m = 0; m < kBoysNM; ++m) tab[(size_t)k * kBoysCols + m] = (double)F[m];
+    tab[(size_t)k * kBoysCols + kBoysNM] = (double)ex;
   }
   return tab;
 }
@@ -338,7 +340,7 @@ int upload_static(sqjk_ctx* c) {
   if ((rc = dalloc(&c->d_sub, c->sub.size()))) return rc;
   if ((rc = dalloc(&c->d_pp, c->pp.size()))) return rc;
   if ((rc = dalloc(&c->d_bscale, (size_t)n))) return rc;
-  if ((rc = dalloc(&c->d_boys, (size_t)kBoysNT * kBoysNM))) return rc;
+  if ((rc = dalloc(&c->d_boys, (size_t)kBoysNT * kBoysCols))) return rc;
   double** mats[] = {&c->d_DJ, &c->d_DA, &c->d_DB, &c->d_AJ, &c->d_AKA, &c->d_AKB,
                      &c->d_J,  &c->d_KA, &c->d_KB, &c->d_sch};
   for (double** m : mats)

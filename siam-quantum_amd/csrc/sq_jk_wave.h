@@ -42,7 +42,7 @@ __device__ __forceinline__ double boys_one(int n, double T, const double* __rest
   if (T < kBoysTMax - 0.5 * kBoysStep) {
     const int k = (int)(T * (1.0 / kBoysStep) + 0.5);
     const double d = k * kBoysStep - T;
-    const double* row = tab + k * kBoysNM + n;
+    const double* row = tab + k * kBoysCols + n;
     double s = row[kBoysTaylor - 1];
 #pragma unroll
     for (int j = kBoysTaylor - 2; j >= 0; --j) s = s * (d * (1.0 / (j + 1))) + row[j];

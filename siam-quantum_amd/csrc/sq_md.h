@@ -63,21 +63,24 @@ constexpr double kTwoPi25 = 34.986836655249725;  // 2 pi^{5/2}
 constexpr double kPrimCutoff = 1.0e-15;            // bare prefactor cut
 
 // ---------------------------------------------------------------- Boys
-// Table: F_m(k*kBoysStep) for k in [0,kBoysNT), m in [0,kBoysNM).
+// Table: F_m(k*kBoysStep) for k in [0,kBoysNT), m in [0,kBoysNM), then
+// exp(-k*kBoysStep) in column kBoysNM (row stride kBoysCols).
 constexpr int kBoysNT = 401;
 constexpr double kBoysStep = 0.1;
 constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 40
 constexpr int kBoysNM = 28;
+constexpr int kBoysCols = kBoysNM + 1;
 constexpr int kBoysTaylor = 9;  // terms of the Taylor expansion
+constexpr int kBoysLdsW = kBoysTaylor + 1;  // LDS row: F_L .. F_{L+8}, exp(-T_k)
 
-// LDS copy keeps the rows m in [0, M) with stride M (M = L + kBoysTaylor).
+// LDS copy for a class of total angular momentum L: per grid point the
+// Taylor coefficients F_L..F_{L+8} and exp(-T_k).
 template <int L>
 __device__ __forceinline__ void boys_load_lds(double* lds, const double* __restrict__ tab) {
-  constexpr int M = L + kBoysTaylor;
-  for (int e = threadIdx.x + threadIdx.y * blockDim.x; e < kBoysNT * M;
+  for (int e = threadIdx.x + threadIdx.y * blockDim.x; e < kBoysNT * kBoysLdsW;
        e += blockDim.x * blockDim.y) {
-    const int k = e / M, m = e - k * M;
-    lds[e] = tab[k * kBoysNM + m];
+    const int k = e / kBoysLdsW, j = e - k * kBoysLdsW;
+    lds[e] = tab[k * kBoysCols + (j < kBoysTaylor ? L + j : kBoysNM)];
   }
 }
 
@@ -94,22 +97,24 @@ __host__ __device__ __forceinline__ double rsqrt_nr(double s) {
   return 1.0 / sqrt(s);
 #endif
 }
+__host__ __device__ __forceinline__ float rsqrt_nr(float s) { return 1.0f / sqrtf(s); }
 
 __host__ __device__ __forceinline__ double rexp(double x) { return exp(x); }
 __host__ __device__ __forceinline__ float rexp(float x) { return expf(x); }
-__host__ __device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
-__host__ __device__ __forceinline__ float rsqrt_(float x) { return sqrtf(x); }
 
 // F_0..F_L at T, evaluated in precision R (the table itself is FP64).
+// T < 39.95: F_L by Taylor expansion about the nearest grid point T_k,
+// exp(-T) = exp(-T_k) exp(T_k - T) with the second factor by its Taylor
+// series (|T_k - T| <= 0.05), then downward recursion.  Larger T: F_0 =
+// sqrt(pi/T)/2 from the refined reciprocal square root, upward recursion.
 // FP32 evaluation needs fewer Taylor terms for the same relative accuracy.
 template <int L, typename R = double>
 __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* lds) {
-  constexpr int M = L + kBoysTaylor;
   constexpr int NT = sizeof(R) == 4 ? 5 : kBoysTaylor;  // 0.05^5/5! ~ 2.6e-9
   if (T < R(kBoysTMax - 0.5 * kBoysStep)) {
     const int k = (int)(T * R(1.0 / kBoysStep) + R(0.5));
     const R d = R(k * kBoysStep) - T;  // expansion in (T_k - T)
-    const double* row = lds + k * M + L;
+    const double* row = lds + k * kBoysLdsW;
     // F_L(T) = sum_j F_{L+j}(T_k) (T_k - T)^j / j!
     R s = R(row[NT - 1]);
     sfor<NT - 1>([&](auto jj) {
@@ -118,7 +123,13 @@ __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* 
     });
     F[L] = s;
     if constexpr (L > 0) {
-      const R e = rexp(-T), t2 = T + T;
+      // exp(d) = sum_j d^j / j!, same number of terms as above
+      R x = R(1);
+      sfor<NT - 1>([&](auto jj) {
+        constexpr int j = NT - 1 - decltype(jj)::value;
+        x = x * (d * R(1.0 / j)) + R(1);
+      });
+      const R e = R(row[kBoysTaylor]) * x, t2 = T + T;
       sfor<L>([&](auto mm) {
         constexpr int m = L - 1 - decltype(mm)::value;
         F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
@@ -126,10 +137,10 @@ __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* 
     }
   } else {
     // large T: F_0 = sqrt(pi/T)/2, upward recursion (stable here)
-    const R rt = R(1) / T;
-    F[0] = R(0.88622692545275801365) * rsqrt_(rt);
+    const R y = rsqrt_nr(T);
+    F[0] = R(0.88622692545275801365) * y;
     if constexpr (L > 0) {
-      const R e = rexp(-T), h = R(0.5) * rt;
+      const R e = rexp(-T), h = R(0.5) * y * y;
       sfor<L>([&](auto mm) {
         constexpr int m = decltype(mm)::value;
         F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;
