@@ -94,6 +94,14 @@ int sqjk_class_stats(const sqjk_ctx *ctx, int *lq, long long *quartets, long lon
 int sqjk_set_profile(sqjk_ctx *ctx, int on);
 int sqjk_class_times(const sqjk_ctx *ctx, double *kernel_ms);
 
+/* One-electron matrices on the device: kinetic T, nuclear attraction V and
+ * overlap S (n x n, row-major; any output may be NULL).  Replaces the three
+ * element loops over GTO_kinetic, GTO_nuclei and GTO_overlap of
+ * src/uhf.c:1307-1334 (same integrals and the same 1e-15 primitive cut of
+ * src/int.c:392-393); the electric-field term is left to the caller. */
+int sqjk_one_electron(int nBasis, const struct GTOBasis_t *gto, const struct Molecule_t *mol, int device,
+                      double *T, double *V, double *S);
+
 /* Number of basis functions, reference shells and pure-l sub-shells. */
 int sqjk_dims(const sqjk_ctx *ctx, int *nBasis, int *nShell, int *nSubShell);
 

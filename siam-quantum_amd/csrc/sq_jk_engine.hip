@@ -35,6 +35,7 @@
 
 #include "../../include/sq_jk.h"
 #include "sq_jk_classes.h"
+#include "sq_one_e.h"
 
 using namespace sqjk;
 
@@ -333,6 +334,13 @@ void plan_pairs(sqjk_ctx* c, const std::vector<double>& pexp, const std::vector<
     }
 }
 
+const std::vector<double>& boys_table() {
+  static std::vector<double> boys;
+  static std::once_flag once;
+  std::call_once(once, [] { boys = make_boys_table(); });
+  return boys;
+}
+
 int upload_static(sqjk_ctx* c) {
   const int n = c->n;
   const size_t nn = (size_t)n * n;
@@ -349,9 +357,7 @@ int upload_static(sqjk_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->d_sub, c->sub.data(), c->sub.size() * sizeof(SubShell), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_bscale, c->bscale.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
-  static std::vector<double> boys;
-  static std::once_flag once;
-  std::call_once(once, [] { boys = make_boys_table(); });
+  const std::vector<double>& boys = boys_table();
   HIPCHK(hipMemcpyAsync(c->d_boys, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
   return SQJK_OK;
 }
@@ -647,6 +653,69 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   }
   if (hipStreamSynchronize(c->st) != hipSuccess) { sqjk_destroy(c); return fail(SQJK_EHIP, "sync failed"); }
   *out = c;
+  return SQJK_OK;
+}
+
+int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, int device, double* T, double* V,
+                      double* S) {
+  if (nBasis <= 0 || !gto || !mol || mol->nAtom < 0) return fail(SQJK_EARG, "sqjk_one_electron: invalid arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(SQJK_ENODEV, "no HIP device is visible; this library has no CPU path");
+  if (device < 0 || device >= ndev) return fail(SQJK_EARG, "sqjk_one_electron: device ordinal out of range");
+  HIPCHK(hipSetDevice(device));
+  sqjk_ctx c;
+  c.n = nBasis;
+  std::vector<double> pexp, pcoef;
+  int rc = plan_shells(&c, nBasis, gto, pexp, pcoef);
+  if (rc) return rc;
+  plan_pairs(&c, pexp, pcoef);
+  std::vector<double> atoms((size_t)4 * std::max(mol->nAtom, 1));
+  for (int k = 0; k < mol->nAtom; ++k) {
+    atoms[4 * k + 0] = mol->x[k]; atoms[4 * k + 1] = mol->y[k]; atoms[4 * k + 2] = mol->z[k];
+    atoms[4 * k + 3] = (double)mol->Z[k];
+  }
+  const size_t nn = (size_t)nBasis * nBasis;
+  std::vector<void*> bufs;
+  auto up = [&](const void* h, size_t bytes) -> void* {
+    void* d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+    bufs.push_back(d);
+    if (h && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+  };
+  auto release = [&] { for (void* d : bufs) (void)hipFree(d); };
+  const std::vector<double>& boys = boys_table();
+  OneEArgs a{};
+  a.sh = (const SubShell*)up(c.sub.data(), c.sub.size() * sizeof(SubShell));
+  a.pexp = (const double*)up(pexp.data(), pexp.size() * 8);
+  a.pcoef = (const double*)up(pcoef.data(), pcoef.size() * 8);
+  a.bscale = (const double*)up(c.bscale.data(), c.bscale.size() * 8);
+  a.atom = (const double*)up(atoms.data(), atoms.size() * 8);
+  a.boys = (const double*)up(boys.data(), boys.size() * 8);
+  a.T = T ? (double*)up(nullptr, nn * 8) : nullptr;
+  a.V = V ? (double*)up(nullptr, nn * 8) : nullptr;
+  a.S = S ? (double*)up(nullptr, nn * 8) : nullptr;
+  a.natom = mol->nAtom;
+  a.n = nBasis;
+  if (!a.sh || !a.pexp || !a.pcoef || !a.bscale || !a.atom || !a.boys || (T && !a.T) || (V && !a.V) || (S && !a.S)) {
+    release();
+    return fail(SQJK_ENOMEM, "sqjk_one_electron: device allocation or upload failed");
+  }
+  for (int X = 0; X < kNPairClass; ++X) {
+    a.pr = (const PairRec*)up(c.pairs[X].data(), c.pairs[X].size() * sizeof(PairRec));
+    a.np = (int)c.pairs[X].size();
+    if (!a.pr || launch_one_electron(X, a, 0) != 0) {
+      release();
+      return fail(SQJK_EHIP, "sqjk_one_electron: kernel launch failed");
+    }
+  }
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess && T) e = hipMemcpy(T, a.T, nn * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && V) e = hipMemcpy(V, a.V, nn * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && S) e = hipMemcpy(S, a.S, nn * 8, hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_one_electron: ") + hipGetErrorString(e));
   return SQJK_OK;
 }
 

@@ -16,9 +16,7 @@
  */
 #include <math.h>
 #include <stdio.h>
-#include <pthread.h>
 #include <stdlib.h>
-#include <unistd.h>
 #include <string.h>
 #include <time.h>
 
@@ -170,37 +168,6 @@ static double total_energy(int n, const struct Molecule_t *mol, const struct opt
 	return E;
 }
 
-/* One-electron matrices H = T + V (+ field) and S, lower triangle rows split
- * over host threads (rows are claimed from a shared counter; the integral
- * routines keep their scratch in thread-local storage). */
-struct one_e_job {
-	int n;
-	const struct GTOBasis_t *gto;
-	const struct Molecule_t *mol;
-	double *H, *S;
-	int next;
-	pthread_mutex_t lock;
-};
-
-static void *one_e_worker(void *arg)
-{
-	struct one_e_job *jb = (struct one_e_job *)arg;
-	const int n = jb->n;
-	for (;;) {
-		pthread_mutex_lock(&jb->lock);
-		const int i = jb->next++;
-		pthread_mutex_unlock(&jb->lock);
-		if (i >= n) break;
-		for (int j = 0; j <= i; j++) {
-			const double h = GTO_kinetic(i, j, jb->gto) + GTO_nuclei(i, j, jb->gto, jb->mol);
-			jb->H[i * n + j] = jb->H[j * n + i] = h;
-			const double s = GTO_overlap(i, j, jb->gto);
-			jb->S[i * n + j] = jb->S[j * n + i] = s;
-		}
-	}
-	return NULL;
-}
-
 static double now_s(void)
 {
 	struct timespec t;
@@ -208,28 +175,20 @@ static double now_s(void)
 	return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
 }
 
-/* host threads: OMP_NUM_THREADS if set, else the online cores, at most 64 */
-static int host_threads(void)
-{
-	const char *env = getenv("OMP_NUM_THREADS");
-	long t = env ? strtol(env, NULL, 10) : sysconf(_SC_NPROCESSORS_ONLN);
-	if (t < 1) t = 1;
-	if (t > 64) t = 64;
-	return (int)t;
-}
-
+/* One-electron matrices H = T + V (+ field) and S (src/uhf.c:1307-1345):
+ * T, V and S on the device (sqjk_one_electron), the field term here. */
 static void one_electron(int n, const struct GTOBasis_t *gto, const struct Molecule_t *mol,
                          const struct option_t *opt, double *H, double *S)
 {
-	struct one_e_job jb = {n, gto, mol, H, S, 0, PTHREAD_MUTEX_INITIALIZER};
-	int nt = host_threads();
-	if (nt > n) nt = n;
-	pthread_t th[64];
-	int started = 0;
-	for (int t = 1; t < nt; t++)
-		if (pthread_create(&th[started], NULL, one_e_worker, &jb) == 0) started++;
-	one_e_worker(&jb);
-	for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+	double *V;
+	ALLOC(V, (size_t)n * n);
+	const int rc = sqjk_one_electron(n, gto, mol, 0, H, V, S);
+	if (rc) {
+		printf("uhf: Error - device one-electron matrices failed (%d): %s\n", rc, sqjk_errmsg());
+		exit(EXIT_FAILURE);
+	}
+	for (size_t e = 0; e < (size_t)n * n; e++) H[e] += V[e];
+	free(V);
 	if (opt->Ex != 0.0 || opt->Ey != 0.0 || opt->Ez != 0.0)
 		for (int i = 0; i < n; i++)
 			for (int j = 0; j < n; j++)

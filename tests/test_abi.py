@@ -21,7 +21,8 @@ def declared_functions():
 
 
 def test_library_exports_declared_symbols():
-    lib = C.CDLL(library_path())
+    # RTLD_NOW: every symbol the library needs must resolve at load time
+    lib = C.CDLL(library_path(), mode=os.RTLD_NOW)
     missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
     assert not missing, missing
 
