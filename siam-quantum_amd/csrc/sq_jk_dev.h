@@ -480,6 +480,7 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
           const int ik = sg.ket0 + (in ? j : lo);  // idle lanes evaluate a valid pair, weight 0
           any = any || in;
           const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
+          if (!__any(valid)) continue;  // the whole wavefront screened out: skip the row (uniform)
           quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, sg.nppk, valid, lds, &run);
         }
         if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, A.sh[A.bra[A.rbra[tl.r0]].a].bf_off, sg.ket0 + j);
