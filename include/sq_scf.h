@@ -77,6 +77,26 @@ double nuclei_coulomb(const struct Molecule_t *mol);
 int get_nEA(struct Molecule_t *mol, int M);
 int get_nEB(struct Molecule_t *mol, int M);
 struct GTOBasisSet_t *read_GAMESS_BasisSet(FILE *inFile, char *name, int *nBasisSet);
+/* ---- checkpoint and density files (src/check.h, src/basis.h) ----
+ * Same text formats as the reference, so files move between the two. */
+void save_checkpoint(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, double Etot, double dE,
+                     double rmsDP, double *CA, double *CB, double *eA, double *eB,
+                     struct option_t *opt);                                   /* src/check.c:432 */
+struct Molecule_t *load_checkpoint_molecule(const struct option_t *opt);      /* src/check.c:37 */
+int load_checkpoint_multiplicity(const struct option_t *opt);                 /* src/check.c:136 */
+struct GTOBasis_t *load_checkpoint_basis(const struct option_t *opt, int *nBasis); /* src/check.c:175 */
+double load_checkpoint_orbital(int nBasis, double *CA, double *CB, double *eA, double *eB,
+                               const struct option_t *opt);                   /* src/check.c:210 */
+void guess_checkpoint_orbital(int nBasis, const struct GTOBasis_t *gto, const struct Molecule_t *mol,
+                              const struct option_t *opt, const double *S, double *CA,
+                              double *CB);                                    /* src/check.c:306 */
+int printGTOBasis(FILE *out, int nBasis, struct GTOBasis_t *gto);             /* src/basis.c:37 */
+struct GTOBasis_t *readGTOInput(FILE *in, int *nBasis);                       /* src/basis.c:84 */
+/* -LDMATRIX / -SVDMATRIX files: n*n values (alpha), then n*n (beta) unless
+ * restricted, whitespace separated (src/uhf.c:1414-1462, 1843-1870). */
+void sq_load_density(int n, int restricted, const char *path, double *PA, double *PB);
+void sq_save_density(int n, const char *path, const double *PA, const double *PB);
+
 struct GTOBasis_t *genBasis(struct Molecule_t *mol, int *nBasis, int dbSize,
                             const struct GTOBasisSet_t *basisDB);
 

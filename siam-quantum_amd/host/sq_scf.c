@@ -260,10 +260,20 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 		}
 		break;
 	}
+	case SCFGUESS_CHECK:
+		/* orbitals of a checkpoint projected on this basis (src/uhf.c:1404-1411) */
+		guess_checkpoint_orbital(n, gto, mol, opt, S, CA, CB);
+		normalize_orbitals(n, nEA, S, CA);
+		normalize_orbitals(n, nEB, S, CB);
+		uhf_getDMatrix(n, nEA, CA, PA);
+		uhf_getDMatrix(n, nEB, CB, PB);
+		break;
 	default:
-		printf("uhf - error this build supports the DIAG, CORE and CACB guesses\n");
+		printf("uhf - error this build supports the DIAG, CORE, CHECK and CACB guesses\n");
 		exit(-1);
 	}
+
+	if (opt->loadDMatrix) sq_load_density(n, opt->restricted, opt->DMatrixFile, PA, PB);  /* src/uhf.c:1414-1462 */
 
 	double sA = 0.0, sB = 0.0;
 	for (size_t e = 0; e < nn; e++) { sA += PA[e]; sB += PB[e]; }
@@ -338,6 +348,7 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			dPB[e] = PB[e] - dPB[e];
 		}
 		tp[5] += now_s() - t0;
+		if (opt->saveCheckAll) save_checkpoint(n, gto, mol, Etot, dE, avgdP, CA, CB, eA, eB, opt);
 		if (iter >= opt->SCFMax) break;
 		if (opt->SCFAccuracy == SCFACCURACY_3STEP) {
 			if (!notConverged) break;
@@ -350,6 +361,10 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			}
 		}
 	} while (notConverged);
+
+	/* density and checkpoint files (src/uhf.c:1843-1883) */
+	if (opt->saveDMatrix) sq_save_density(n, opt->DMatrixFile, PA, PB);
+	if (opt->saveCheck) save_checkpoint(n, gto, mol, Etot, dE, avgdP, CA, CB, eA, eB, opt);
 
 	/* release the J/K state (the reference resets its statics here) */
 	{

@@ -64,7 +64,7 @@ class Option(C.Structure):
 
 CONVMETHOD_DIIS4 = 3
 SCFACCURACY_3STEP, SCFACCURACY_1STEP = 0, 1
-SCFGUESS_DIAG, SCFGUESS_CORE, SCFGUESS_CACB = 0, 1, 3
+SCFGUESS_DIAG, SCFGUESS_CORE, SCFGUESS_CHECK, SCFGUESS_CACB = 0, 1, 2, 3
 
 
 def default_option(**kw):

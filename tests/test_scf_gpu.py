@@ -9,6 +9,7 @@ import pytest
 
 from molecules import g2, water
 from sqamd import Molecule, GTOBasis, Option, default_option, dptr, lib
+from sqamd.abi import SCFGUESS_CHECK
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -71,3 +72,27 @@ def test_uhf_doublet_radical(oracle):
                        three_step=False, use_mpole=False, max_mem=0)
     print(f"UHF CH3: E(gpu) = {E:.10f}  E(oracle) = {Eo:.10f}  log = {GOLD['ch3_q0m2']['631g_star']['energy']}")
     assert abs(E - Eo) < 1e-8
+
+
+def _iterations(err):
+    line = [s for s in err.splitlines() if s.startswith("uhf timing")][-1]
+    return int(line.rsplit("(", 1)[1].split()[0])
+
+
+def test_checkpoint_and_density_restart(tmp_path, monkeypatch, capfd):
+    """-SAVECHECK then -GUESS=CHECK (src/check.c, src/uhf.c:1282-1287), and
+    -SVDMATRIX then -LDMATRIX (src/uhf.c:1414-1462, 1843-1870): restarting
+    from the converged state reaches the same energy in fewer iterations."""
+    monkeypatch.setenv("SQ_SCF_TIMING", "1")
+    mol, bas = g2("h2o_q0m1", "631g_star")
+    chk, dm = str(tmp_path / "w.chk").encode(), str(tmp_path / "w.dm").encode()
+    E0 = _uhf(mol, bas, 5, 5, SCFConv=1e-10, SCFAccuracy=1, saveCheck=1, CheckFile=chk, saveDMatrix=1,
+              DMatrixFile=dm)
+    n0 = _iterations(capfd.readouterr().err)
+    E1 = _uhf(mol, bas, 5, 5, SCFConv=1e-10, SCFAccuracy=1, SCFGuess=SCFGUESS_CHECK, CheckFile=chk)
+    n1 = _iterations(capfd.readouterr().err)
+    E2 = _uhf(mol, bas, 5, 5, SCFConv=1e-10, SCFAccuracy=1, loadDMatrix=1, DMatrixFile=dm)
+    n2 = _iterations(capfd.readouterr().err)
+    print(f"E = {E0:.10f} ({n0} it)  CHECK {E1:.10f} ({n1} it)  LDMATRIX {E2:.10f} ({n2} it)")
+    assert abs(E1 - E0) < 1e-9 and abs(E2 - E0) < 1e-9
+    assert n1 < n0 and n2 < n0
