@@ -63,6 +63,33 @@ struct Tile {
 
 enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
 
+// Load of read-only work-list data through the constant address space: the
+// compiler may then assume the memory is not written during the kernel, so a
+// wavefront-uniform address (the bra rows and their primitive pairs, tiles,
+// segments) becomes one scalar load instead of a per-lane vector load plus
+// readfirstlane.  A lane-varying address still compiles to a vector load.
+// The record is read as words (16, 8 or 4 bytes by its alignment), since an
+// aggregate copy would be lowered back to global-memory loads.
+template <class T>
+__host__ __device__ __forceinline__ T uload(const T* p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  using W = typename std::conditional<
+      alignof(T) >= 16, unsigned __attribute__((ext_vector_type(4))),
+      typename std::conditional<alignof(T) >= 8, unsigned __attribute__((ext_vector_type(2))), unsigned>::type>::type;
+  static_assert(sizeof(T) % sizeof(W) == 0, "record size must be a multiple of its load word");
+  constexpr int N = sizeof(T) / sizeof(W);
+  const __attribute__((address_space(4))) W* q = (const __attribute__((address_space(4))) W*)p;
+  W w[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = q[i];
+  T t;
+  __builtin_memcpy(&t, w, sizeof(T));
+  return t;
+#else
+  return *p;
+#endif
+}
+
 struct ClassArgs {
   const SubShell* __restrict__ sh;
   const PairRec* __restrict__ bra;
@@ -169,9 +196,9 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
   constexpr int LAB = LA + LB, L = LAB + LC + LD;
   constexpr int NAB = NA * NB, NS = CD1 - CD0;
 
-  const PairRec bp = A.bra[ib];
+  const PairRec bp = uload(A.bra + ib);
   const PairRec kp = A.ket[ik];
-  const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
+  const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = A.sh[kp.a], sD = A.sh[kp.b];
 
   R acc[NAB][NS];
   sfor<NAB>([&](auto a) { sfor<NS>([&](auto c) { acc[a][c] = R(0); }); });
@@ -186,7 +213,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     herm_coef<LC, LD, R>(Fy, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
     herm_coef<LC, LD, R>(Fz, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
     for (int u = 0; u < nppb; ++u) {
-      const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
+      const PrimPair P = uload(A.pp + (bp.pp_off + u * bp.pp_stride));
       const double s = P.p + Q.p;
       const double rs = rsqrt_nr(s);
       if constexpr (MODE != kSchwarz) {
@@ -428,7 +455,7 @@ __host__ __device__ __forceinline__ bool density_screen_pass(const ClassArgs& A,
                                                              const PairRec& kp) {
   const double* PM = A.PM;
   const int nr = A.nref;
-  const double mp = PM[bp.pa * nr + bp.pb] + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
+  const double mp = uload(PM + (bp.pa * nr + bp.pb)) + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
                     PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
   return !(bp.sch * kp.sch * mp < A.fixed_cut);
 }
@@ -467,23 +494,23 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
       const long long wstride = (long long)gridDim.x * wpb;
       const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
       for (long long w = (long long)blockIdx.x * wpb + wib; w < A.total; w += wstride) {
-        const Tile tl = A.tile[w];
-        const Segment sg = A.seg[tl.seg];
+        const Tile tl = uload(A.tile + w);
+        const Segment sg = uload(A.seg + tl.seg);
         const int j = tl.k0 + lane;  // ket position in the group
         RunAcc<LA, LC, LD, CD0, CD1> run;
         run.zero();
         bool any = false;
         for (int r = tl.r0; r < tl.r1; ++r) {
-          const int ib = A.rbra[r];
-          const int lo = A.rlo[r];
-          const bool in = j >= lo && j < A.rhi[r];
+          const int ib = uload(A.rbra + r);
+          const int lo = uload(A.rlo + r);
+          const bool in = j >= lo && j < uload(A.rhi + r);
           const int ik = sg.ket0 + (in ? j : lo);  // idle lanes evaluate a valid pair, weight 0
           any = any || in;
-          const bool valid = in && density_screen_pass(A, A.bra[ib], A.ket[ik]);
+          const bool valid = in && density_screen_pass(A, uload(A.bra + ib), A.ket[ik]);
           if (!__any(valid)) continue;  // the whole wavefront screened out: skip the row (uniform)
           quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, sg.nppk, valid, lds, &run);
         }
-        if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, A.sh[A.bra[A.rbra[tl.r0]].a].bf_off, sg.ket0 + j);
+        if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, uload(A.sh + (uload(A.bra + (uload(A.rbra + tl.r0))).a)).bf_off, sg.ket0 + j);
       }
     }
   });

@@ -108,10 +108,10 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
   for (long long t = blockIdx.x; t < items; t += gridDim.x) {
     int r0 = 0, r1 = 1, j = 0, nppb = 0, nppk = 0, ket0 = 0;
     if constexpr (MODE == kSchwarz) {
-      nppb = nppk = A.bra[t].npp;
+      nppb = nppk = uload(A.bra + t).npp;
     } else {
-      const Tile tl = A.tile[t >> 6];
-      const Segment sg = A.seg[tl.seg];
+      const Tile tl = uload(A.tile + (t >> 6));
+      const Segment sg = uload(A.seg + tl.seg);
       r0 = tl.r0; r1 = tl.r1; j = tl.k0 + (int)(t & 63);
       nppb = sg.nppb; nppk = sg.nppk; ket0 = sg.ket0;
     }
@@ -120,26 +120,26 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
     if constexpr (MODE == kSchwarz) {
       ib = ik = (int)t;
     } else {
-      if (j < A.rlo[r] || j >= A.rhi[r]) continue;                  // uniform
-      ib = A.rbra[r];
+      if (j < uload(A.rlo + r) || j >= uload(A.rhi + r)) continue;  // uniform
+      ib = uload(A.rbra + r);
       ik = ket0 + j;
-      if (!density_screen_pass(A, A.bra[ib], A.ket[ik])) continue;   // uniform
+      if (!density_screen_pass(A, uload(A.bra + ib), uload(A.ket + ik))) continue;  // uniform
     }
-    const PairRec bp = A.bra[ib], kp = A.ket[ik];
-    const SubShell sA = A.sh[bp.a], sB = A.sh[bp.b], sC = A.sh[kp.a], sD = A.sh[kp.b];
+    const PairRec bp = uload(A.bra + ib), kp = uload(A.ket + ik);
+    const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = uload(A.sh + kp.a), sD = uload(A.sh + kp.b);
 
     R acc[W_::NOUT];
 #pragma unroll
     for (int k = 0; k < W_::NOUT; ++k) acc[k] = R(0);
 
     for (int u = 0; u < nppb; ++u) {
-      const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
+      const PrimPair P = uload(A.pp + (bp.pp_off + u * bp.pp_stride));
       for (int v = 0; v < nppk; ++v) {
-        const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
+        const PrimPair Q = uload(A.pp + (kp.pp_off + v * kp.pp_stride));
         const double s = P.p + Q.p;
         const double rs = rsqrt_nr(s);
         if constexpr (MODE != kSchwarz) {
-          if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;   // uniform
+          if (kTwoPi25 * P.eb * Q.eb * rs < kPrimCutoff) continue;  // uniform
         }
         const double alpha_d = P.p * Q.p * (rs * rs);
         const double Xd = P.Px - Q.Px, Yd = P.Py - Q.Py, Zd = P.Pz - Q.Pz;
