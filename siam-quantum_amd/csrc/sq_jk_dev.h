@@ -516,13 +516,16 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
   });
 }
 
-// Workgroup size of the per-thread kernel: 1024 lanes (16 wavefronts share
-// one LDS Boys table, up to 128 VGPRs per lane) for classes holding at most 9
-// integrals per lane, else 256.
+// Workgroup size of the per-thread kernel: 512 lanes (8 wavefronts share one
+// LDS Boys table) for classes holding at most 9 integrals per lane, else 256.
+// These kernels keep about 110 SGPRs per wavefront, so a SIMD holds at most 7
+// wavefronts: with 512-lane workgroups (2 wavefronts per SIMD each) a CU
+// keeps 3 workgroups, 6 wavefronts per SIMD, where 1024-lane workgroups
+// would leave it at one workgroup and 4.
 template <int LA, int LB, int LC, int LD>
 constexpr int class_block() {
   using SI = SliceInfo<LA, LB, LC, LD>;
-  return SI::NAB * SI::SZ <= 9 ? 1024 : 256;
+  return SI::NAB * SI::SZ <= 9 ? 512 : 256;
 }
 
 template <int LA, int LB, int LC, int LD>
