@@ -1,6 +1,8 @@
 // sq_jk_class.hip - kernels and launcher of one angular class, selected at
 // compile time with -DSQ_LA=.. -DSQ_LB=.. -DSQ_LC=.. -DSQ_LD=.. (the build
 // compiles this file once per class, in parallel).
+#include <algorithm>
+
 #include "sq_jk_classes.h"
 #include "sq_jk_wave.h"
 
@@ -48,10 +50,32 @@ int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
+// Workgroups of the per-thread kernel that the current device holds at once
+// (occupancy x CUs), computed once per kernel.
+template <int MODE, typename R>
+long long resident_blocks() {
+  static const long long v = [] {
+    constexpr size_t lds = class_lds_bytes<LA, LB, LC, LD>();
+    constexpr int bs = class_block<LA, LB, LC, LD>();
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+      ncu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_class<LA, LB, LC, LD, MODE, R, bs>, bs, lds) !=
+            hipSuccess || nb < 1)
+      nb = 1;
+    return (long long)nb * ncu;
+  }();
+  return v;
+}
+
 // grid <= 0: size the grid from the work, a.total = tiles (RHF/UHF) or pairs
 // (Schwarz).  Per-thread kernel: one wavefront per tile (one lane per pair)
-// and slice, at most 2^21 lanes per slice; wave kernel: one 64-lane
-// workgroup per tile or pair, at most 2^20.
+// and slice, at most a few times the workgroups the device holds at once
+// over all slices, so each workgroup fills its LDS Boys table once and then
+// strides over many tiles (4x for the 512-lane classes, whose tiles differ
+// most in cost, so that later workgroups even out the end; 2x otherwise);
+// wave kernel: one 64-lane workgroup per tile or pair, at most 2^20.
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
@@ -61,7 +85,9 @@ int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   } else {
     constexpr int bs = class_block<LA, LB, LC, LD>();
     const long long lanes = MODE == kSchwarz ? a.total : a.total * 64;
-    const long long b = (lanes + bs - 1) / bs, bmax = (1 << 21) / bs;
+    const long long b = (lanes + bs - 1) / bs;
+    constexpr long long over = bs >= 512 ? 4 : 2;
+    const long long bmax = std::max(1LL, over * resident_blocks<MODE, R>() / SI::NSLICE);
     const long long g = grid > 0 ? grid : (b < bmax ? b : bmax);
     return launch_thread<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   }

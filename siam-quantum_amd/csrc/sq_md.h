@@ -66,9 +66,9 @@ constexpr double kPrimCutoff = 1.0e-15;            // bare prefactor cut
 // ---------------------------------------------------------------- Boys
 // Table: F_m(k*kBoysStep) for k in [0,kBoysNT), m in [0,kBoysNM), then
 // exp(-k*kBoysStep) in column kBoysNM (row stride kBoysCols).
-constexpr int kBoysNT = 401;
+constexpr int kBoysNT = 601;
 constexpr double kBoysStep = 0.1;
-constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 40
+constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 60
 constexpr int kBoysNM = 28;
 constexpr int kBoysCols = kBoysNM + 1;
 constexpr int kBoysTaylor = 9;  // terms of the Taylor expansion
@@ -100,15 +100,17 @@ __host__ __device__ __forceinline__ double rsqrt_nr(double s) {
 }
 __host__ __device__ __forceinline__ float rsqrt_nr(float s) { return 1.0f / sqrtf(s); }
 
-__host__ __device__ __forceinline__ double rexp(double x) { return exp(x); }
-__host__ __device__ __forceinline__ float rexp(float x) { return expf(x); }
 
 // F_0..F_L at T, evaluated in precision R (the table itself is FP64).
-// T < 39.95: F_L by Taylor expansion about the nearest grid point T_k,
+// T < 59.95: F_L by Taylor expansion about the nearest grid point T_k,
 // exp(-T) = exp(-T_k) exp(T_k - T) with the second factor by its Taylor
 // series (|T_k - T| <= 0.05), then downward recursion.  Larger T: F_0 =
-// sqrt(pi/T)/2 from the refined reciprocal square root, upward recursion.
-// FP32 evaluation needs fewer Taylor terms for the same relative accuracy.
+// sqrt(pi/T)/2 from the refined reciprocal square root (erfc(sqrt(T)) <
+// 1e-27), and the upward recursion F_{m+1} = (2m+1) F_m / (2T) without the
+// exp(-T) term, which changes F_m (m <= 8) by less than 2e-17 relative for
+// T >= 59.95, so no exponential is evaluated on this branch.  The table reaches T = 60 so that
+// the branch needs none.  FP32 evaluation needs fewer Taylor terms for the
+// same relative accuracy.
 template <int L, typename R = double>
 __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* lds) {
   constexpr int NT = sizeof(R) == 4 ? 5 : kBoysTaylor;  // 0.05^5/5! ~ 2.6e-9
@@ -137,14 +139,14 @@ __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* 
       });
     }
   } else {
-    // large T: F_0 = sqrt(pi/T)/2, upward recursion (stable here)
+    // large T: F_0 = sqrt(pi/T)/2, F_{m+1} = (2m+1) F_m / (2T)
     const R y = rsqrt_nr(T);
     F[0] = R(0.88622692545275801365) * y;
     if constexpr (L > 0) {
-      const R e = rexp(-T), h = R(0.5) * y * y;
+      const R h = R(0.5) * y * y;
       sfor<L>([&](auto mm) {
         constexpr int m = decltype(mm)::value;
-        F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;
+        F[m + 1] = (R(2 * m + 1) * h) * F[m];
       });
     }
   }
