@@ -120,6 +120,32 @@ __global__ void k_finalize(const double* __restrict__ AJ, const double* __restri
   }
 }
 
+// D_J = P_A + P_B
+__global__ void k_density_sum(const double* __restrict__ DA, const double* __restrict__ DB, double* DJ, size_t nn) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x)
+    DJ[e] = DA[e] + DB[e];
+}
+
+// PM[P][Q] = max |D| over the (P,Q) block of reference shells for D = D_J,
+// P_A, P_B (the density factor of the screening test, src/matrix.c:3921-3924)
+__global__ void k_pair_max(const double* __restrict__ DJ, const double* __restrict__ DA,
+                           const double* __restrict__ DB, const int* __restrict__ ref, int nr, int n,
+                           double* PM) {
+  const long long tot = (long long)nr * nr;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int P = (int)(t / nr), Q = (int)(t - (long long)P * nr);
+    const int p0 = ref[2 * P], np = ref[2 * P + 1], q0 = ref[2 * Q], nq = ref[2 * Q + 1];
+    double r = 0.0;
+    for (int p = p0; p < p0 + np; ++p)
+      for (int q = q0; q < q0 + nq; ++q) {
+        const size_t e = (size_t)p * n + q;
+        r = fmax(r, fmax(fabs(DJ[e]), fmax(fabs(DA[e]), fabs(DB[e]))));
+      }
+    PM[t] = r;
+  }
+}
+
 // A_J[fa+ia][fb+ib] += Jp[ab][i] for every pair i of one class.  Every
 // (pair, component) owns one element of A_J, so plain read-modify-write is
 // safe (the class kernels have finished).
@@ -217,7 +243,9 @@ struct sqjk_ctx {
   double* d_SB[kNPairClass] = {};
   size_t jp_len[kNPairClass] = {}, s_len[kNPairClass] = {};
   double mixed_thr = 0.0;  // Schwarz products below this run in FP32
-  std::vector<double> h_DJ, h_PM, h_J, h_KA, h_KB;
+  // pinned host staging of the densities (in) and of J, K (out)
+  double *hp_A = nullptr, *hp_B = nullptr, *hp_J = nullptr, *hp_KA = nullptr, *hp_KB = nullptr;
+  int* d_ref = nullptr;  // reference shells: first basis function, count
   sqjk_stats last{};
   // per class pair (X,Y) of the last build: contracted and primitive quartets
   long long cls_q[kNPairClass][kNPairClass] = {};
@@ -354,6 +382,19 @@ int upload_static(sqjk_ctx* c) {
   for (double** m : mats)
     if ((rc = dalloc(m, nn))) return rc;
   if ((rc = dalloc(&c->d_PM, (size_t)c->nref * c->nref))) return rc;
+  if ((rc = dalloc(&c->d_ref, (size_t)2 * c->nref))) return rc;
+  double** pins[] = {&c->hp_A, &c->hp_B, &c->hp_J, &c->hp_KA, &c->hp_KB};
+  for (double** h : pins)
+    if (hipHostMalloc((void**)h, nn * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      *h = nullptr;
+      return fail(SQJK_ENOMEM, "pinned host allocation failed");
+    }
+  std::vector<int> ref2((size_t)2 * c->nref);
+  for (int P = 0; P < c->nref; ++P) {
+    ref2[2 * P] = c->ref[P].first;
+    ref2[2 * P + 1] = c->ref[P].count;
+  }
+  HIPCHK(hipMemcpy(c->d_ref, ref2.data(), ref2.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpyAsync(c->d_sub, c->sub.data(), c->sub.size() * sizeof(SubShell), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_bscale, c->bscale.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
@@ -727,26 +768,18 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   const int n = c->n, nr = c->nref;
   const size_t nn = (size_t)n * n;
   const int uhf = restricted ? 0 : 1;
-  // densities: J uses PA+PB, K uses PA (and PB)
-  c->h_DJ.resize(nn);
-  for (size_t e = 0; e < nn; ++e) c->h_DJ[e] = PA[e] + PB[e];
-  c->h_PM.assign((size_t)nr * nr, 0.0);
-  for (int P = 0; P < nr; ++P)
-    for (int Q = 0; Q < nr; ++Q) {
-      double r = 0.0;
-      for (int p = c->ref[P].first; p < c->ref[P].first + c->ref[P].count; ++p)
-        for (int q = c->ref[Q].first; q < c->ref[Q].first + c->ref[Q].count; ++q) {
-          const size_t e = (size_t)p * n + q;
-          r = std::max(r, std::fabs(c->h_DJ[e]));
-          r = std::max(r, std::fabs(PA[e]));
-          r = std::max(r, std::fabs(PB[e]));
-        }
-      c->h_PM[(size_t)P * nr + Q] = r;
-    }
-  HIPCHK(hipMemcpyAsync(c->d_DJ, c->h_DJ.data(), nn * 8, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->d_DA, PA, nn * 8, hipMemcpyHostToDevice, c->st));
-  if (uhf) HIPCHK(hipMemcpyAsync(c->d_DB, PB, nn * 8, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->d_PM, c->h_PM.data(), (size_t)nr * nr * 8, hipMemcpyHostToDevice, c->st));
+  // densities: J uses PA+PB, K uses PA (and PB).  Both go up through the
+  // pinned staging buffers (the previous build has synchronised, so they are
+  // free); D_J and the screening maxima are formed on the device.
+  std::memcpy(c->hp_A, PA, nn * sizeof(double));
+  std::memcpy(c->hp_B, PB, nn * sizeof(double));
+  HIPCHK(hipMemcpyAsync(c->d_DA, c->hp_A, nn * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
+                     c->d_DJ, nn);
+  hipLaunchKernelGGL(k_pair_max, dim3(grid_for((long long)nr * nr, 256)), dim3(256), 0, c->st, c->d_DJ, c->d_DA,
+                     c->d_DB, c->d_ref, nr, n, c->d_PM);
+  HIPCHK(hipGetLastError());
 
   // work plan (rows/segments), rebuilt only when the cut-offs change
   const double thr = std::max(fixedCutoff, scfCutoff);
@@ -823,23 +856,27 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, c->d_J,
                      c->d_KA, c->d_KB, n, uhf);
   HIPCHK(hipGetLastError());
-  c->h_J.resize(nn); c->h_KA.resize(nn); c->h_KB.resize(nn);
-  HIPCHK(hipMemcpyAsync(c->h_J.data(), c->d_J, nn * 8, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(c->h_KA.data(), c->d_KA, nn * 8, hipMemcpyDeviceToHost, c->st));
-  if (uhf) HIPCHK(hipMemcpyAsync(c->h_KB.data(), c->d_KB, nn * 8, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->hp_J, c->d_J, nn * 8, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->hp_KA, c->d_KA, nn * 8, hipMemcpyDeviceToHost, c->st));
+  if (uhf) HIPCHK(hipMemcpyAsync(c->hp_KB, c->d_KB, nn * 8, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  // reference semantics: accumulate, KB := KA if restricted, upper := lower
-  for (size_t e = 0; e < nn; ++e) {
-    JT[e] += c->h_J[e];
-    KA[e] += c->h_KA[e];
-    if (uhf) KB[e] += c->h_KB[e];
-  }
-  if (!uhf) std::memcpy(KB, KA, nn * sizeof(double));
-  for (int p = 0; p < n; ++p)
-    for (int q = 0; q < p; ++q) {
-      JT[(size_t)q * n + p] = JT[(size_t)p * n + q];
-      KA[(size_t)q * n + p] = KA[(size_t)p * n + q];
-      KB[(size_t)q * n + p] = KB[(size_t)p * n + q];
+  // reference semantics: accumulate into the lower triangle, KB := KA if
+  // restricted, upper := lower; done in 64 x 64 blocks so that the mirrored
+  // writes stay in cache
+  const double* hK2 = uhf ? c->hp_KB : c->hp_KA;
+  constexpr int kBlk = 64;
+  for (int p0 = 0; p0 < n; p0 += kBlk)
+    for (int q0 = 0; q0 <= p0; q0 += kBlk) {
+      const int p1 = std::min(n, p0 + kBlk), q1 = std::min(n, q0 + kBlk);
+      for (int p = p0; p < p1; ++p)
+        for (int q = q0; q < std::min(q1, p + 1); ++q) {
+          const size_t e = (size_t)p * n + q, t = (size_t)q * n + p;
+          const double j = JT[e] + c->hp_J[e], ka = KA[e] + c->hp_KA[e];
+          const double kb = uhf ? KB[e] + hK2[e] : ka;
+          JT[e] = JT[t] = j;
+          KA[e] = KA[t] = ka;
+          KB[e] = KB[t] = kb;
+        }
     }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -921,6 +958,10 @@ void sqjk_destroy(sqjk_ctx* c) {
                   c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_tile, c->d_rbra, c->d_rlo, c->d_rhi};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->d_ref) (void)hipFree(c->d_ref);
+  double* pins[] = {c->hp_A, c->hp_B, c->hp_J, c->hp_KA, c->hp_KB};
+  for (double* h : pins)
+    if (h) (void)hipHostFree(h);
   for (int X = 0; X < kNPairClass; ++X) {
     if (c->d_pairs[X]) (void)hipFree(c->d_pairs[X]);
     if (c->d_Jp[X]) (void)hipFree(c->d_Jp[X]);
