@@ -2,7 +2,7 @@
 //
 // One 64-lane workgroup (one wavefront) owns one contracted shell quartet at
 // a time.  Per primitive quartet the lanes build, in LDS,
-//   R_{TUV}   (closed form, one entry per lane),
+//   R_{TUV}   (level recursion, a few entries per lane and level),
 //   E^{ab}, E^{cd} Hermite coefficients (one axis per lane),
 //   W[cd][tuv] = sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V},
 // and every lane accumulates its share of the (ab|cd) block in registers.
@@ -29,11 +29,6 @@ __host__ __device__ constexpr HermTab make_herm_tab() {
   return h;
 }
 __constant__ constexpr HermTab kHerm = make_herm_tab();
-
-// d^T/dX^T of g(X^2):  sum_i c(T,i) (2X)^(T-2i) g^(T-i),  c = T!/(i!(T-2i)!)
-__constant__ constexpr double kHermC[9][5] = {
-    {1, 0, 0, 0, 0},   {1, 0, 0, 0, 0},   {1, 2, 0, 0, 0},    {1, 6, 0, 0, 0},     {1, 12, 12, 0, 0},
-    {1, 20, 60, 0, 0}, {1, 30, 180, 120, 0}, {1, 42, 420, 840, 0}, {1, 56, 840, 3360, 1680}};
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }  // 64-thread workgroups
 
@@ -80,27 +75,59 @@ struct WaveLayout {
   static constexpr int NW = NCD * NHAB;
   static constexpr int NBIG = NW > NINT ? NW : NINT;
   // LDS offsets (doubles)
-  static constexpr int oR = 0, oEa = oR + NH, oEc = oEa + 3 * EA, oW = oEc + 3 * EC;
+  static constexpr int oR = 0, oR2 = oR + NH, oEa = oR2 + NH, oEc = oEa + 3 * EA, oW = oEc + 3 * EC;
   static constexpr int oD = oW + NBIG;                 // staged density blocks
   static constexpr int ND6 = NAB + NCD + NB * ND + NA * ND + NB * NC + NA * NC;
-  static constexpr int TOTAL = oD + 2 * ND6;
-  static constexpr int NOUT = (NINT + 63) / 64;         // integrals per lane
+  static constexpr int oH = oD + 2 * ND6;              // (t,u,v) of the Hermite index, one int each
+  static constexpr int oAcc = oH + (NH + 1) / 2;       // contracted (ab|cd) block, lane-owned entries
+  static constexpr int TOTAL = oAcc + NINT;
+  static constexpr int NSL = (NH + 63) / 64;           // R entries per lane
 };
 
 // R: precision of the integral evaluation; staging of the contracted block
 // and the digestion are FP64.
 template <int LA, int LB, int LC, int LD, int MODE, typename R = double>
-__global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
+__global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
   using W_ = WaveLayout<LA, LB, LC, LD>;
   constexpr int NA = W_::NA, NB = W_::NB, NC = W_::NC, ND = W_::ND;
   constexpr int NAB = W_::NAB, NCD = W_::NCD, NINT = W_::NINT, L = W_::L, LAB = W_::LAB;
   __shared__ double lds[W_::TOTAL];
   R* Rh = reinterpret_cast<R*>(lds + W_::oR);    // views sized in doubles,
-  R* Ea = reinterpret_cast<R*>(lds + W_::oEa);   // so an FP32 view fits
+  R* Rh2 = reinterpret_cast<R*>(lds + W_::oR2);  // so an FP32 view fits
+  R* Ea = reinterpret_cast<R*>(lds + W_::oEa);
   R* Ec = reinterpret_cast<R*>(lds + W_::oEc);
   R* Wt = reinterpret_cast<R*>(lds + W_::oW);
+  int* hdec = reinterpret_cast<int*>(lds + W_::oH);
+  // Each lane accumulates the entries o = lane + 64 k of the block in LDS
+  // (runtime loops over them keep the register count low).
+  R* acc = reinterpret_cast<R*>(lds + W_::oAcc);
   const int lane = threadIdx.x;
   const int n = A.n;
+
+  // Hermite index decode table in LDS (t | u << 8 | v << 16), and per lane
+  // the recursion step of each R entry it owns, e = lane + 64 s:
+  //   R^n_{tuv} = P_ax R^{n+1}_{..-1..} + c2 R^{n+1}_{..-2..}
+  // along the first axis with a nonzero index (ax = -1 for e = 0).
+  for (int e = lane; e < W_::NH; e += 64) hdec[e] = kHerm.t[e] | (kHerm.u[e] << 8) | (kHerm.v[e] << 16);
+  int r_ax[W_::NSL], r_i1[W_::NSL], r_i2[W_::NSL], r_ord[W_::NSL];
+  R r_c2[W_::NSL];
+#pragma unroll
+  for (int sl = 0; sl < W_::NSL; ++sl) {
+    const int e = lane + 64 * sl;
+    r_ax[sl] = -1; r_i1[sl] = 0; r_i2[sl] = 0; r_c2[sl] = R(0); r_ord[sl] = 1 << 20;
+    if (e < W_::NH) {
+      const int t = kHerm.t[e], u = kHerm.u[e], v = kHerm.v[e];
+      r_ord[sl] = t + u + v;
+      if (t > 0) {
+        r_ax[sl] = 0; r_i1[sl] = hidx(t - 1, u, v); r_i2[sl] = t > 1 ? hidx(t - 2, u, v) : 0; r_c2[sl] = R(t - 1);
+      } else if (u > 0) {
+        r_ax[sl] = 1; r_i1[sl] = hidx(t, u - 1, v); r_i2[sl] = u > 1 ? hidx(t, u - 2, v) : 0; r_c2[sl] = R(u - 1);
+      } else if (v > 0) {
+        r_ax[sl] = 2; r_i1[sl] = hidx(t, u, v - 1); r_i2[sl] = v > 1 ? hidx(t, u, v - 2) : 0; r_c2[sl] = R(v - 1);
+      }
+    }
+  }
+  wave_sync();
 
   // RHF/UHF: grid-stride over (tile, ket position) items, the rows of the
   // tile one after the other; Schwarz: grid-stride over the pairs.
@@ -128,9 +155,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
     const PairRec bp = uload(A.bra + ib), kp = uload(A.ket + ik);
     const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = uload(A.sh + kp.a), sD = uload(A.sh + kp.b);
 
-    R acc[W_::NOUT];
-#pragma unroll
-    for (int k = 0; k < W_::NOUT; ++k) acc[k] = R(0);
+    for (int o = lane; o < NINT; o += 64) acc[o] = R(0);
 
     for (int u = 0; u < nppb; ++u) {
       const PrimPair P = uload(A.pp + (bp.pp_off + u * bp.pp_stride));
@@ -147,48 +172,46 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
         const R X = R(Xd), Y = R(Yd), Z = R(Zd);
         const R pref = R(kTwoPi25 * P.cc * Q.cc * rs);
 
-        // g_m = (-alpha)^m F_m(T) for m <= L, one lane each, via LDS scratch
+        // g_m = (-2 alpha)^m F_m(T) = R^m_{000} for m <= L, one lane each, via LDS scratch
         R* g = Wt;  // W is rebuilt below, so its space is free here
         if (lane <= L) {
           double pw = 1.0;
-          for (int m = 0; m < lane; ++m) pw *= -alpha_d;
+          for (int m = 0; m < lane; ++m) pw *= -2.0 * alpha_d;
           g[lane] = R(pw * boys_one(lane, T, A.boys));
         }
         const R h2p = R(P.hp), h2q = R(Q.hp);
         if (lane == 60) herm_axis_to_lds<LA, LB, R>(Ea + 0 * W_::EA, R(P.Px - sA.x), R(P.Px - sB.x), h2p);
         if (lane == 61) herm_axis_to_lds<LA, LB, R>(Ea + 1 * W_::EA, R(P.Py - sA.y), R(P.Py - sB.y), h2p);
         if (lane == 62) herm_axis_to_lds<LA, LB, R>(Ea + 2 * W_::EA, R(P.Pz - sA.z), R(P.Pz - sB.z), h2p);
-        if (lane == 63) {
-          herm_axis_to_lds<LC, LD, R>(Ec + 0 * W_::EC, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
-          herm_axis_to_lds<LC, LD, R>(Ec + 1 * W_::EC, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
-          herm_axis_to_lds<LC, LD, R>(Ec + 2 * W_::EC, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
-        }
+        if (lane == 57) herm_axis_to_lds<LC, LD, R>(Ec + 0 * W_::EC, R(Q.Px - sC.x), R(Q.Px - sD.x), h2q);
+        if (lane == 58) herm_axis_to_lds<LC, LD, R>(Ec + 1 * W_::EC, R(Q.Py - sC.y), R(Q.Py - sD.y), h2q);
+        if (lane == 59) herm_axis_to_lds<LC, LD, R>(Ec + 2 * W_::EC, R(Q.Pz - sC.z), R(Q.Pz - sD.z), h2q);
         wave_sync();
-        // R_{TUV} = sum_{ijk} c(T,i)(2X)^{T-2i} c(U,j)(2Y)^{U-2j} c(V,k)(2Z)^{V-2k} g_{T+U+V-i-j-k}
-        for (int e = lane; e < W_::NH; e += 64) {
-          const int tT = kHerm.t[e], tU = kHerm.u[e], tV = kHerm.v[e];
-          R r = R(0);
-          for (int i = 0; 2 * i <= tT; ++i) {
-            R px = R(kHermC[tT][i]);
-            for (int q = 0; q < tT - 2 * i; ++q) px *= R(2) * X;
-            for (int j = 0; 2 * j <= tU; ++j) {
-              R py = px * R(kHermC[tU][j]);
-              for (int q = 0; q < tU - 2 * j; ++q) py *= R(2) * Y;
-              for (int k = 0; 2 * k <= tV; ++k) {
-                R pz = py * R(kHermC[tV][k]);
-                for (int q = 0; q < tV - 2 * k; ++q) pz *= R(2) * Z;
-                r += pz * g[tT + tU + tV - i - j - k];
-              }
+        // R^0_{tuv} by the McMurchie-Davidson recursion over the levels
+        // n = L-1 .. 0 (level n holds the orders up to L-n), alternating
+        // between two LDS buffers so that level 0 lands in Rh
+#pragma unroll 1
+        for (int lev = L - 1; lev >= 0; --lev) {
+          // level L holds R^L_{000} = g_L only, which the order-1 entries of
+          // level L-1 read (their second index is 0 with weight 0)
+          const R* prev = (lev == L - 1) ? g + L : (((lev + 1) & 1) ? Rh2 : Rh);
+          R* next = (lev & 1) ? Rh2 : Rh;
+#pragma unroll
+          for (int sl = 0; sl < W_::NSL; ++sl) {
+            const int e = lane + 64 * sl;
+            if (e < W_::NH && r_ord[sl] <= L - lev) {
+              const int ax = r_ax[sl];
+              const R pa = ax == 0 ? X : (ax == 1 ? Y : Z);
+              next[e] = ax < 0 ? g[lev] : pa * prev[r_i1[sl]] + r_c2[sl] * prev[r_i2[sl]];
             }
           }
-          Rh[e] = r;
+          wave_sync();
         }
-        wave_sync();
         // W[cd][tuv]
         for (int e = lane; e < W_::NW; e += 64) {
           const int cd = e / W_::NHAB, h = e - cd * W_::NHAB;
           const Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
-          const int t0 = kHerm.t[h], u0 = kHerm.u[h], v0 = kHerm.v[h];
+          const int hd = hdec[h], t0 = hd & 255, u0 = (hd >> 8) & 255, v0 = hd >> 16;
           const R* ex = Ec + 0 * W_::EC + (c.x * (LD + 1) + d.x) * (LC + LD + 1);
           const R* ey = Ec + 1 * W_::EC + (c.y * (LD + 1) + d.y) * (LC + LD + 1);
           const R* ez = Ec + 2 * W_::EC + (c.z * (LD + 1) + d.z) * (LC + LD + 1);
@@ -206,10 +229,9 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
         }
         wave_sync();
         // (ab|cd) += pref sum_{tuv} E^{ab}_{tuv} W[cd][tuv]
-#pragma unroll
-        for (int k = 0; k < W_::NOUT; ++k) {
-          const int o = lane + 64 * k;
-          if (o < NINT) {
+#pragma unroll 1
+        for (int o = lane; o < NINT; o += 64) {
+          {
             const int ab = o / NCD, cd = o - ab * NCD;
             const Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
             const R* ex = Ea + 0 * W_::EA + (a.x * (LB + 1) + b.x) * (LAB + 1);
@@ -223,7 +245,7 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
                 for (int v_ = 0; v_ <= a.z + b.z; ++v_) vz += ez[v_] * w[hidx(t_, u_, v_)];
                 val += ex[t_] * ey[u_] * vz;
               }
-            acc[k] += pref * val;
+            acc[o] += pref * val;
           }
         }
         wave_sync();
@@ -232,15 +254,13 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
 
     const int fa = sA.bf_off, fb = sB.bf_off, fc = sC.bf_off, fd = sD.bf_off;
     if constexpr (MODE == kSchwarz) {
-#pragma unroll
-      for (int k = 0; k < W_::NOUT; ++k) {
-        const int o = lane + 64 * k;
-        if (o < NINT) {
+      for (int o = lane; o < NINT; o += 64) {
+        {
           const int ab = o / NCD, cd = o - ab * NCD;
           if (ab == cd) {
             const int ia = ab / NB, jb = ab % NB;
             const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
-            const double v = sqrt(fabs((double)acc[k] * sc * sc));
+            const double v = sqrt(fabs((double)acc[o] * sc * sc));
             A.sch_out[(fa + ia) * n + fb + jb] = v;
             A.sch_out[(fb + jb) * n + fa + ia] = v;
           }
@@ -252,12 +272,10 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       if (kp.a == kp.b) deg *= 0.5;
       if (A.same && ib == ik) deg *= 0.5;
       double* EE = lds + W_::oW;
-#pragma unroll
-      for (int k = 0; k < W_::NOUT; ++k) {
-        const int o = lane + 64 * k;
-        if (o < NINT) {
+      for (int o = lane; o < NINT; o += 64) {
+        {
           const int ab = o / NCD, cd = o - ab * NCD;
-          EE[o] = (double)acc[k] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
+          EE[o] = (double)acc[o] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
                   A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
         }
       }
@@ -270,10 +288,12 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
       for (int e = lane; e < NAB + NCD; e += 64) {
         double s = 0.0;
         if (e < NAB) {
+#pragma unroll 6
           for (int cd = 0; cd < NCD; ++cd) s += EE[e * NCD + cd] * Dst[NAB + cd];
           fatomic(&A.AJ[(fa + e / NB) * n + fb + e % NB], 2.0 * s);
         } else {
           const int cd = e - NAB;
+#pragma unroll 6
           for (int ab = 0; ab < NAB; ++ab) s += EE[ab * NCD + cd] * Dst[ab];
           fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * s);
         }
@@ -298,21 +318,25 @@ __global__ void __launch_bounds__(64) k_class_wave(ClassArgs A) {
           double s = 0.0;
           if (e < NA * NC) {                       // K[a,c] += sum_bd v D[b,d]
             const int a = e / NC, c = e % NC;
+#pragma unroll 1
             for (int b = 0; b < NB; ++b)
               for (int d = 0; d < ND; ++d) s += EE[(a * NB + b) * NCD + c * ND + d] * dbd[b * ND + d];
             fatomic(&AK[(fa + a) * n + fc + c], s);
           } else if (e < NA * NC + NB * NC) {      // K[b,c] += sum_ad v D[a,d]
             const int r = e - NA * NC, b = r / NC, c = r % NC;
+#pragma unroll 1
             for (int a = 0; a < NA; ++a)
               for (int d = 0; d < ND; ++d) s += EE[(a * NB + b) * NCD + c * ND + d] * dad[a * ND + d];
             fatomic(&AK[(fb + b) * n + fc + c], s);
           } else if (e < NA * NC + NB * NC + NA * ND) {  // K[a,d] += sum_bc v D[b,c]
             const int r = e - NA * NC - NB * NC, a = r / ND, d = r % ND;
+#pragma unroll 1
             for (int b = 0; b < NB; ++b)
               for (int c = 0; c < NC; ++c) s += EE[(a * NB + b) * NCD + c * ND + d] * dbc[b * NC + c];
             fatomic(&AK[(fa + a) * n + fd + d], s);
           } else {                                 // K[b,d] += sum_ac v D[a,c]
             const int r = e - NA * NC - NB * NC - NA * ND, b = r / ND, d = r % ND;
+#pragma unroll 1
             for (int a = 0; a < NA; ++a)
               for (int c = 0; c < NC; ++c) s += EE[(a * NB + b) * NCD + c * ND + d] * dac[a * NC + c];
             fatomic(&AK[(fb + b) * n + fd + d], s);
