@@ -507,17 +507,31 @@ void finish_pairs(sqjk_ctx* c) {
 // Schwarz product >= thr64 run in FP64, those in [thr, thr64) in FP32.
 // Bra rows are taken in (first shell, Schwarz) order; a tile is up to
 // kTileRows rows of one first shell that meet one chunk of 64 ket positions.
-// Host only; make_plan uploads the result.  SQJK_TILE_ROWS (1-256) sets the
-// rows per tile.  Default 1: on (H2O)32/6-31G* one row per tile measured
-// 22.8 ms per build, 4 rows 23.5 ms, 32 rows 40.2 ms (longer serial tiles cost
-// more than the register-held run contributions save).
-int tile_rows() {
-  static const int v = [] {
+// Host only; make_plan uploads the result.  Rows per tile come from
+// tile_rows(X, Y) below.
+// Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
+// dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
+// in registers across both rows and halve their atomics; that pays where the
+// bra's second shell is s (and for (pp|ss), (dp|ss)), while the classes with
+// larger bra blocks lose more to register pressure than they save.  Measured
+// on MI355X, (H2O)32 6-31G*: 17.4 ms per build with one row everywhere, 16.2
+// ms with two rows everywhere.  SQJK_TILE_ROWS overrides the table.
+int tile_rows(int X, int Y) {
+  static const int env = [] {
     const char* e = std::getenv("SQJK_TILE_ROWS");
-    const int r = e ? std::atoi(e) : 1;
-    return r < 1 ? 1 : (r > 256 ? 256 : r);
+    const int r = e ? std::atoi(e) : 0;
+    return r < 0 ? 0 : (r > 256 ? 256 : r);
   }();
-  return v;
+  if (env > 0) return env;
+  static const int kRows[kNPairClass][kNPairClass] = {
+      // Y: ss ps pp ds dp dd
+      {2, 0, 0, 0, 0, 0},   // X = ss
+      {2, 2, 0, 0, 0, 0},   // X = ps
+      {2, 1, 1, 0, 0, 0},   // X = pp
+      {2, 2, 2, 2, 0, 0},   // X = ds
+      {2, 1, 1, 1, 1, 0},   // X = dp
+      {1, 1, 1, 1, 1, 1}};  // X = dd
+  return kRows[X][Y] > 0 ? kRows[X][Y] : 1;
 }
 
 void build_plan(sqjk_ctx* c, double thr, double thr64) {
@@ -589,7 +603,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                 sel.clear();
                 for (size_t q = r; q < e; ++q)
                   if (rows[q].lo < k0 + 64 && rows[q].hi > k0) sel.push_back(rows[q]);
-                const size_t tr = (size_t)tile_rows();
+                const size_t tr = (size_t)tile_rows(X, Y);
                 for (size_t q = 0; q < sel.size(); q += tr) {
                   const int r0 = (int)c->h_rbra.size();
                   for (size_t u = q; u < std::min(sel.size(), q + tr); ++u) {
