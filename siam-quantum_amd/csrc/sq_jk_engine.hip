@@ -512,10 +512,11 @@ void finish_pairs(sqjk_ctx* c) {
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
 // in registers across both rows and halve their atomics; that pays where the
-// bra's second shell is s (and for (pp|ss), (dp|ss)), while the classes with
-// larger bra blocks lose more to register pressure than they save.  Measured
-// on MI355X, (H2O)32 6-31G*: 17.4 ms per build with one row everywhere, 16.2
-// ms with two rows everywhere.  SQJK_TILE_ROWS overrides the table.
+// bra's second shell is s and for most dp bras, while (pp|pp), (pp|ps),
+// (dp|pp) and the dd bras lose more to register pressure than they save.
+// Measured per class at 1, 2 and 3 rows on MI355X, (H2O)32 6-31G*: 16.7 ms
+// per build with one row everywhere, 14.8 ms with two rows everywhere.
+// SQJK_TILE_ROWS overrides the table.
 int tile_rows(int X, int Y) {
   static const int env = [] {
     const char* e = std::getenv("SQJK_TILE_ROWS");
@@ -529,7 +530,7 @@ int tile_rows(int X, int Y) {
       {2, 2, 0, 0, 0, 0},   // X = ps
       {2, 1, 1, 0, 0, 0},   // X = pp
       {2, 2, 2, 2, 0, 0},   // X = ds
-      {2, 1, 1, 1, 1, 0},   // X = dp
+      {2, 2, 1, 2, 2, 0},   // X = dp
       {1, 1, 1, 1, 1, 1}};  // X = dd
   return kRows[X][Y] > 0 ? kRows[X][Y] : 1;
 }
