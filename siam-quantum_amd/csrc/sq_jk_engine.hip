@@ -172,7 +172,11 @@ struct KStage {
 };
 
 // One workgroup per row x of A_K: the row is summed in LDS, then added to A_K.
-__global__ void __launch_bounds__(256) k_reduce_k(KStage R, const SubShell* __restrict__ sh, double* AK, int n) {
+// The staging is streamed with four independent loads per lane in flight
+// (1024-lane workgroups), since the reduction is bound by HBM reads.
+constexpr int kReduceKBlock = 1024;
+__global__ void __launch_bounds__(kReduceKBlock) k_reduce_k(KStage R, const SubShell* __restrict__ sh, double* AK,
+                                                            int n) {
   extern __shared__ double row[];
   for (int x = blockIdx.x; x < n; x += gridDim.x) {
     for (int y = threadIdx.x; y < n; y += blockDim.x) row[y] = 0.0;
@@ -183,13 +187,20 @@ __global__ void __launch_bounds__(256) k_reduce_k(KStage R, const SubShell* __re
       const double* Sx = R.S[Y] + (size_t)x * ncomp * np;
       const PairRec* prY = R.pr[Y];
       const long long tot = (long long)ncomp * np;
-      for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
-        const double v = Sx[e];
-        if (v == 0.0) continue;
-        const int comp = (int)(e / np), j = (int)(e - (long long)comp * np);
-        const PairRec p = prY[j];
-        const int col = comp < nc ? sh[p.a].bf_off + comp : sh[p.b].bf_off + comp - nc;
-        __hip_atomic_fetch_add(&row[col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const long long bs = blockDim.x;
+      for (long long e0 = threadIdx.x; e0 < tot; e0 += 4 * bs) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (e0 + k * bs < tot) ? Sx[e0 + k * bs] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (v[k] == 0.0) continue;
+          const long long e = e0 + k * bs;
+          const int comp = (int)(e / np), j = (int)(e - (long long)comp * np);
+          const PairRec p = prY[j];
+          const int col = comp < nc ? sh[p.a].bf_off + comp : sh[p.b].bf_off + comp - nc;
+          __hip_atomic_fetch_add(&row[col], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
       }
     }
     __syncthreads();
@@ -864,8 +875,11 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
     RA.nc[X] = RB.nc[X] = sqmd::ncart(kPairL[X][0]);
     RA.ncomp[X] = RB.ncomp[X] = sqmd::ncart(kPairL[X][0]) + nb;
   }
-  hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(256), (size_t)n * sizeof(double), c->st, RA, c->d_sub, c->d_AKA, n);
-  if (uhf) hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(256), (size_t)n * sizeof(double), c->st, RB, c->d_sub, c->d_AKB, n);
+  hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RA, c->d_sub,
+                     c->d_AKA, n);
+  if (uhf)
+    hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RB, c->d_sub,
+                       c->d_AKB, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev1, c->st));
   hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, c->d_J,
