@@ -611,7 +611,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                 mhi = std::max(mhi, rows[e].hi);
                 ++e;
               }
-              for (int k0 = mlo / 64 * 64; k0 < mhi; k0 += 64) {
+              for (int k0 = mlo; k0 < mhi; k0 += 64) {  // chunks start at the run's first ket
                 sel.clear();
                 for (size_t q = r; q < e; ++q)
                   if (rows[q].lo < k0 + 64 && rows[q].hi > k0) sel.push_back(rows[q]);
