@@ -570,7 +570,14 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
         if (fp == 1 && !(thr64c > thr)) continue;
         const auto &bra = c->pairs[X], &ket = c->pairs[Y];
         sqjk_ctx::Launch L{X, Y, fp, (int)c->h_seg.size(), (int)c->h_tile.size(), 0, 0};
-        for (size_t a = 0; a < c->groups[X].size(); ++a)
+        for (size_t a = 0; a < c->groups[X].size(); ++a) {
+          // bra rows of group a in (first shell, Schwarz) order, shared by all ket groups
+          {
+            const auto& g1 = c->groups[X][a];
+            order.resize(g1.count);
+            for (int k = 0; k < g1.count; ++k) order[k] = g1.begin + k;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bra[x].a < bra[y].a; });
+          }
           for (size_t b = 0; b < c->groups[Y].size(); ++b) {
             if (X == Y && b > a) continue;  // canonical: ket pair index <= bra pair index
             const auto &g1 = c->groups[X][a], &g2 = c->groups[Y][b];
@@ -583,9 +590,6 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
               }
               return lo;
             };
-            order.resize(g1.count);
-            for (int k = 0; k < g1.count; ++k) order[k] = g1.begin + k;
-            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bra[x].a < bra[y].a; });
             rows.clear();
             long long acc = 0;
             for (int i : order) {
@@ -631,6 +635,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
             L.quartets += acc;
             c->cls_p[X][Y] += acc * g1.npp * g2.npp;
           }
+        }
         L.ntile = (long long)c->h_tile.size() - L.tile0;
         if (L.ntile == 0) continue;
         c->launches.push_back(L);
