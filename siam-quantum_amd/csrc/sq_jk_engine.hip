@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -520,6 +521,18 @@ void finish_pairs(sqjk_ctx* c) {
 // kTileRows rows of one first shell that meet one chunk of 64 ket positions.
 // Host only; make_plan uploads the result.  Rows per tile come from
 // tile_rows(X, Y) below.
+// Runs fn(t, nt) on nt host threads (t = 0..nt-1), nt = min(maxt, hardware
+// threads); the calling thread takes t = 0.
+template <class F>
+void host_parallel(int maxt, F&& fn) {
+  static const int hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = std::max(1, std::min(maxt, hw));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(fn, t, nt);
+  fn(0, nt);
+  for (auto& th : pool) th.join();
+}
+
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
 // in registers across both rows and halve their atomics; that pays where the
@@ -802,8 +815,11 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   // densities: J uses PA+PB, K uses PA (and PB).  Both go up through the
   // pinned staging buffers (the previous build has synchronised, so they are
   // free); D_J and the screening maxima are formed on the device.
-  std::memcpy(c->hp_A, PA, nn * sizeof(double));
-  std::memcpy(c->hp_B, PB, nn * sizeof(double));
+  host_parallel(8, [&](int t, int nt) {
+    const size_t e0 = nn * t / nt, e1 = nn * (t + 1) / nt;
+    std::memcpy(c->hp_A + e0, PA + e0, (e1 - e0) * sizeof(double));
+    std::memcpy(c->hp_B + e0, PB + e0, (e1 - e0) * sizeof(double));
+  });
   HIPCHK(hipMemcpyAsync(c->d_DA, c->hp_A, nn * 8, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
   hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
@@ -899,19 +915,24 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   // writes stay in cache
   const double* hK2 = uhf ? c->hp_KB : c->hp_KA;
   constexpr int kBlk = 64;
-  for (int p0 = 0; p0 < n; p0 += kBlk)
-    for (int q0 = 0; q0 <= p0; q0 += kBlk) {
+  const int nblk = (n + kBlk - 1) / kBlk;
+  // block rows dealt out round-robin; the (p0, q0) blocks and their mirrors
+  // are disjoint, so threads never write the same element
+  host_parallel(8, [&](int t, int nt) {
+  for (int pb = t; pb < nblk; pb += nt)
+    for (int q0 = 0, p0 = pb * kBlk; q0 <= p0; q0 += kBlk) {
       const int p1 = std::min(n, p0 + kBlk), q1 = std::min(n, q0 + kBlk);
       for (int p = p0; p < p1; ++p)
         for (int q = q0; q < std::min(q1, p + 1); ++q) {
-          const size_t e = (size_t)p * n + q, t = (size_t)q * n + p;
+          const size_t e = (size_t)p * n + q, tr = (size_t)q * n + p;
           const double j = JT[e] + c->hp_J[e], ka = KA[e] + c->hp_KA[e];
           const double kb = uhf ? KB[e] + hK2[e] : ka;
-          JT[e] = JT[t] = j;
-          KA[e] = KA[t] = ka;
-          KB[e] = KB[t] = kb;
+          JT[e] = JT[tr] = j;
+          KA[e] = KA[tr] = ka;
+          KB[e] = KB[tr] = kb;
         }
     }
+  });
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   std::memset(c->cls_ms, 0, sizeof(c->cls_ms));
