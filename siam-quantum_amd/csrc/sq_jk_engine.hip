@@ -536,8 +536,9 @@ void host_parallel(int maxt, F&& fn) {
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
 // in registers across both rows and halve their atomics; that pays where the
-// bra's second shell is s and for most dp bras, while (pp|pp), (pp|ps),
-// (dp|pp) and the dd bras lose more to register pressure than they save.
+// bra's second shell is s and for most dp bras (three for (ps|ps) and the ds
+// bras), while (pp|pp), (pp|ps), (dp|pp) and the dd bras lose more to
+// register pressure than they save.
 // Measured per class at 1, 2 and 3 rows on MI355X, (H2O)32 6-31G*: 16.7 ms
 // per build with one row everywhere, 14.8 ms with two rows everywhere.
 // SQJK_TILE_ROWS overrides the table.
@@ -551,9 +552,9 @@ int tile_rows(int X, int Y) {
   static const int kRows[kNPairClass][kNPairClass] = {
       // Y: ss ps pp ds dp dd
       {2, 0, 0, 0, 0, 0},   // X = ss
-      {2, 2, 0, 0, 0, 0},   // X = ps
+      {2, 3, 0, 0, 0, 0},   // X = ps
       {2, 1, 1, 0, 0, 0},   // X = pp
-      {2, 2, 2, 2, 0, 0},   // X = ds
+      {3, 3, 3, 3, 0, 0},   // X = ds
       {2, 2, 1, 2, 2, 0},   // X = dp
       {1, 1, 1, 1, 1, 1}};  // X = dd
   return kRows[X][Y] > 0 ? kRows[X][Y] : 1;
