@@ -46,7 +46,7 @@ int launch_thread(const ClassArgs& a, int gx, hipStream_t st) {
 
 template <int MODE, typename R>
 int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_class_wave<LA, LB, LC, LD, MODE, R>), dim3(grid), dim3(64), 0, st, a);
+  hipLaunchKernelGGL((k_class_wave<LA, LB, LC, LD, MODE, R>), dim3(grid), dim3(kWT), 0, st, a);
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
@@ -75,7 +75,8 @@ long long resident_blocks() {
 // over all slices, so each workgroup fills its LDS Boys table once and then
 // strides over many tiles (4x for the 512-lane classes, whose tiles differ
 // most in cost, so that later workgroups even out the end; 2x otherwise);
-// wave kernel: one 64-lane workgroup per tile or pair, at most 2^20.
+// wave kernel: one 128-lane workgroup per (tile, ket position) or pair, at
+// most 2^20.
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {

@@ -1,11 +1,11 @@
 // sq_jk_wave.h - cooperative kernel for the high-angular-momentum classes.
 //
-// One 64-lane workgroup (one wavefront) owns one contracted shell quartet at
-// a time.  Per primitive quartet the lanes build, in LDS,
+// One 128-lane workgroup (two wavefronts) owns one contracted shell quartet
+// at a time.  Per primitive quartet the lanes build, in LDS,
 //   R_{TUV}   (level recursion, a few entries per lane and level),
 //   E^{ab}, E^{cd} Hermite coefficients (one axis per lane),
 //   W[cd][tuv] = sum_{TUV} (-1)^{T+U+V} E^{cd}_{TUV} R_{t+T,u+U,v+V},
-// and every lane accumulates its share of the (ab|cd) block in registers.
+// and every lane accumulates its share of the (ab|cd) block in LDS.
 // The contracted block is then staged in LDS and digested into J/K with one
 // atomic per J/K element touched by the quartet.
 #pragma once
@@ -30,7 +30,10 @@ __host__ __device__ constexpr HermTab make_herm_tab() {
 }
 __constant__ constexpr HermTab kHerm = make_herm_tab();
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }  // 64-thread workgroups
+// Threads of the cooperative workgroup (two wavefronts on one quartet).
+constexpr int kWT = 128;
+
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // F_n(T) for one n straight from the global table (Taylor about the grid point)
 __device__ __forceinline__ double boys_one(int n, double T, const double* __restrict__ tab) {
@@ -81,13 +84,13 @@ struct WaveLayout {
   static constexpr int oH = oD + 2 * ND6;              // (t,u,v) of the Hermite index, one int each
   static constexpr int oAcc = oH + (NH + 1) / 2;       // contracted (ab|cd) block, lane-owned entries
   static constexpr int TOTAL = oAcc + NINT;
-  static constexpr int NSL = (NH + 63) / 64;           // R entries per lane
+  static constexpr int NSL = (NH + kWT - 1) / kWT;     // R entries per lane
 };
 
 // R: precision of the integral evaluation; staging of the contracted block
 // and the digestion are FP64.
 template <int LA, int LB, int LC, int LD, int MODE, typename R = double>
-__global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
+__global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
   using W_ = WaveLayout<LA, LB, LC, LD>;
   constexpr int NA = W_::NA, NB = W_::NB, NC = W_::NC, ND = W_::ND;
   constexpr int NAB = W_::NAB, NCD = W_::NCD, NINT = W_::NINT, L = W_::L, LAB = W_::LAB;
@@ -98,22 +101,22 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
   R* Ec = reinterpret_cast<R*>(lds + W_::oEc);
   R* Wt = reinterpret_cast<R*>(lds + W_::oW);
   int* hdec = reinterpret_cast<int*>(lds + W_::oH);
-  // Each lane accumulates the entries o = lane + 64 k of the block in LDS
+  // Each lane accumulates the entries o = lane + kWT k of the block in LDS
   // (runtime loops over them keep the register count low).
   R* acc = reinterpret_cast<R*>(lds + W_::oAcc);
   const int lane = threadIdx.x;
   const int n = A.n;
 
   // Hermite index decode table in LDS (t | u << 8 | v << 16), and per lane
-  // the recursion step of each R entry it owns, e = lane + 64 s:
+  // the recursion step of each R entry it owns, e = lane + kWT s:
   //   R^n_{tuv} = P_ax R^{n+1}_{..-1..} + c2 R^{n+1}_{..-2..}
   // along the first axis with a nonzero index (ax = -1 for e = 0).
-  for (int e = lane; e < W_::NH; e += 64) hdec[e] = kHerm.t[e] | (kHerm.u[e] << 8) | (kHerm.v[e] << 16);
+  for (int e = lane; e < W_::NH; e += kWT) hdec[e] = kHerm.t[e] | (kHerm.u[e] << 8) | (kHerm.v[e] << 16);
   int r_ax[W_::NSL], r_i1[W_::NSL], r_i2[W_::NSL], r_ord[W_::NSL];
   R r_c2[W_::NSL];
 #pragma unroll
   for (int sl = 0; sl < W_::NSL; ++sl) {
-    const int e = lane + 64 * sl;
+    const int e = lane + kWT * sl;
     r_ax[sl] = -1; r_i1[sl] = 0; r_i2[sl] = 0; r_c2[sl] = R(0); r_ord[sl] = 1 << 20;
     if (e < W_::NH) {
       const int t = kHerm.t[e], u = kHerm.u[e], v = kHerm.v[e];
@@ -155,7 +158,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
     const PairRec bp = uload(A.bra + ib), kp = uload(A.ket + ik);
     const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = uload(A.sh + kp.a), sD = uload(A.sh + kp.b);
 
-    for (int o = lane; o < NINT; o += 64) acc[o] = R(0);
+    for (int o = lane; o < NINT; o += kWT) acc[o] = R(0);
 
     for (int u = 0; u < nppb; ++u) {
       const PrimPair P = uload(A.pp + (bp.pp_off + u * bp.pp_stride));
@@ -198,7 +201,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
           R* next = (lev & 1) ? Rh2 : Rh;
 #pragma unroll
           for (int sl = 0; sl < W_::NSL; ++sl) {
-            const int e = lane + 64 * sl;
+            const int e = lane + kWT * sl;
             if (e < W_::NH && r_ord[sl] <= L - lev) {
               const int ax = r_ax[sl];
               const R pa = ax == 0 ? X : (ax == 1 ? Y : Z);
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
           wave_sync();
         }
         // W[cd][tuv]
-        for (int e = lane; e < W_::NW; e += 64) {
+        for (int e = lane; e < W_::NW; e += kWT) {
           const int cd = e / W_::NHAB, h = e - cd * W_::NHAB;
           const Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
           const int hd = hdec[h], t0 = hd & 255, u0 = (hd >> 8) & 255, v0 = hd >> 16;
@@ -230,7 +233,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
         wave_sync();
         // (ab|cd) += pref sum_{tuv} E^{ab}_{tuv} W[cd][tuv]
 #pragma unroll 1
-        for (int o = lane; o < NINT; o += 64) {
+        for (int o = lane; o < NINT; o += kWT) {
           {
             const int ab = o / NCD, cd = o - ab * NCD;
             const Cart a = cart(LA, ab / NB), b = cart(LB, ab % NB);
@@ -254,7 +257,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
 
     const int fa = sA.bf_off, fb = sB.bf_off, fc = sC.bf_off, fd = sD.bf_off;
     if constexpr (MODE == kSchwarz) {
-      for (int o = lane; o < NINT; o += 64) {
+      for (int o = lane; o < NINT; o += kWT) {
         {
           const int ab = o / NCD, cd = o - ab * NCD;
           if (ab == cd) {
@@ -272,7 +275,7 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
       if (kp.a == kp.b) deg *= 0.5;
       if (A.same && ib == ik) deg *= 0.5;
       double* EE = lds + W_::oW;
-      for (int o = lane; o < NINT; o += 64) {
+      for (int o = lane; o < NINT; o += kWT) {
         {
           const int ab = o / NCD, cd = o - ab * NCD;
           EE[o] = (double)acc[o] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
@@ -282,10 +285,10 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
       wave_sync();
       // Coulomb
       double* Dst = lds + W_::oD;
-      for (int e = lane; e < NAB; e += 64) Dst[e] = A.DJ[(fa + e / NB) * n + fb + e % NB];
-      for (int e = lane; e < NCD; e += 64) Dst[NAB + e] = A.DJ[(fc + e / ND) * n + fd + e % ND];
+      for (int e = lane; e < NAB; e += kWT) Dst[e] = A.DJ[(fa + e / NB) * n + fb + e % NB];
+      for (int e = lane; e < NCD; e += kWT) Dst[NAB + e] = A.DJ[(fc + e / ND) * n + fd + e % ND];
       wave_sync();
-      for (int e = lane; e < NAB + NCD; e += 64) {
+      for (int e = lane; e < NAB + NCD; e += kWT) {
         double s = 0.0;
         if (e < NAB) {
 #pragma unroll 6
@@ -308,13 +311,13 @@ __global__ void __launch_bounds__(64, 2) k_class_wave(ClassArgs A) {
         double* dbc = dad + NA * ND;          // D[b,c]  NB x NC
         double* dac = dbc + NB * NC;          // D[a,c]  NA x NC
         wave_sync();
-        for (int e = lane; e < NB * ND; e += 64) dbd[e] = D[(fb + e / ND) * n + fd + e % ND];
-        for (int e = lane; e < NA * ND; e += 64) dad[e] = D[(fa + e / ND) * n + fd + e % ND];
-        for (int e = lane; e < NB * NC; e += 64) dbc[e] = D[(fb + e / NC) * n + fc + e % NC];
-        for (int e = lane; e < NA * NC; e += 64) dac[e] = D[(fa + e / NC) * n + fc + e % NC];
+        for (int e = lane; e < NB * ND; e += kWT) dbd[e] = D[(fb + e / ND) * n + fd + e % ND];
+        for (int e = lane; e < NA * ND; e += kWT) dad[e] = D[(fa + e / ND) * n + fd + e % ND];
+        for (int e = lane; e < NB * NC; e += kWT) dbc[e] = D[(fb + e / NC) * n + fc + e % NC];
+        for (int e = lane; e < NA * NC; e += kWT) dac[e] = D[(fa + e / NC) * n + fc + e % NC];
         wave_sync();
         constexpr int NK = NA * NC + NB * NC + NA * ND + NB * ND;
-        for (int e = lane; e < NK; e += 64) {
+        for (int e = lane; e < NK; e += kWT) {
           double s = 0.0;
           if (e < NA * NC) {                       // K[a,c] += sum_bd v D[b,d]
             const int a = e / NC, c = e % NC;
