@@ -816,13 +816,22 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   // densities: J uses PA+PB, K uses PA (and PB).  Both go up through the
   // pinned staging buffers (the previous build has synchronised, so they are
   // free); D_J and the screening maxima are formed on the device.
-  host_parallel(8, [&](int t, int nt) {
-    const size_t e0 = nn * t / nt, e1 = nn * (t + 1) / nt;
-    std::memcpy(c->hp_A + e0, PA + e0, (e1 - e0) * sizeof(double));
-    std::memcpy(c->hp_B + e0, PB + e0, (e1 - e0) * sizeof(double));
-  });
+  // The copy of P_B into its staging buffer overlaps the upload of P_A; a
+  // caller passing the same matrix twice gets one upload and a device copy.
+  auto stage = [&](double* dst, const double* src) {
+    host_parallel(8, [&](int t, int nt) {
+      const size_t e0 = nn * t / nt, e1 = nn * (t + 1) / nt;
+      std::memcpy(dst + e0, src + e0, (e1 - e0) * sizeof(double));
+    });
+  };
+  stage(c->hp_A, PA);
   HIPCHK(hipMemcpyAsync(c->d_DA, c->hp_A, nn * 8, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
+  if (PB == PA) {
+    HIPCHK(hipMemcpyAsync(c->d_DB, c->d_DA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  } else {
+    stage(c->hp_B, PB);
+    HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
+  }
   hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
                      c->d_DJ, nn);
   hipLaunchKernelGGL(k_pair_max, dim3(grid_for((long long)nr * nr, 256)), dim3(256), 0, c->st, c->d_DJ, c->d_DA,
