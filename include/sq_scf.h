@@ -39,6 +39,13 @@ int linear_solver(int nRow, int nCol, const double *A, double *B);
 int sqla_gen_eigen(int n, const double *F, const double *S, double *e, double *C);
 /* P = sum_{k<nOcc} C_k C_k^T (C orbital-major) as one device DGEMM; 0 on success. */
 int sqla_density(int n, int nOcc, const double *C, double *P);
+/* Density of the lowest nOcc solutions of F C = S C e straight from F
+ * (spectral projection on the device, no eigenvectors).  Returns 0 on
+ * success, 1 when the projection does not settle (no gap at nOcc; P is then
+ * untouched and the caller solves the eigenproblem), < 0 on device errors.
+ * No counterpart in the reference, whose SCF forms P from the orbitals of
+ * gen_sym_SomeEigen (src/uhf.c:1703-1710); uhf() uses it for the same P. */
+int sqla_density_sp2(int n, int nOcc, const double *F, const double *S, double *P);
 
 /* ---- Boys function (src/fgamma.h:19-24) ---- */
 double fgamma_0(double x);

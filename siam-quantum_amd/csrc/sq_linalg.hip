@@ -30,6 +30,7 @@ struct EigWork {
   int n = 0;
   rocblas_handle h = nullptr;
   double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr, *P = nullptr, *X = nullptr;
+  double* red = nullptr;  // Gershgorin bounds and traces of the purification
   rocblas_int* info = nullptr;
   int metric_n = 0;             // X = U^-1 is valid for this n and the host copy below
   std::vector<double> metric;   // S of the cached factor
@@ -42,14 +43,14 @@ int ensure(int n) {
   if (g_w.n >= n) return 0;
   if (g_w.A) {
     (void)hipFree(g_w.A); (void)hipFree(g_w.B); (void)hipFree(g_w.W); (void)hipFree(g_w.E);
-    (void)hipFree(g_w.P); (void)hipFree(g_w.X); (void)hipFree(g_w.info);
+    (void)hipFree(g_w.P); (void)hipFree(g_w.X); (void)hipFree(g_w.info); (void)hipFree(g_w.red);
   }
   g_w.metric_n = 0;
   const size_t nn = (size_t)n * n;
   if (hipMalloc(&g_w.A, nn * 8) != hipSuccess || hipMalloc(&g_w.B, nn * 8) != hipSuccess ||
       hipMalloc(&g_w.P, nn * 8) != hipSuccess || hipMalloc(&g_w.X, nn * 8) != hipSuccess ||
       hipMalloc(&g_w.W, (size_t)n * 8) != hipSuccess || hipMalloc(&g_w.E, (size_t)n * 8) != hipSuccess ||
-      hipMalloc(&g_w.info, sizeof(rocblas_int)) != hipSuccess) {
+      hipMalloc(&g_w.info, sizeof(rocblas_int)) != hipSuccess || hipMalloc(&g_w.red, 4 * sizeof(double)) != hipSuccess) {
     g_w.n = 0;
     return -1;
   }
@@ -64,6 +65,72 @@ __global__ void k_zero_lower(double* X, int n) {
     const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
     if (i > j) X[e] = 0.0;
   }
+}
+
+// One workgroup: red[0] = min_i (F_ii - sum_{j!=i} |F_ij|), red[1] = max_i
+// (F_ii + sum_{j!=i} |F_ij|), Gershgorin bounds of the spectrum of F.
+constexpr int kRedBlock = 1024;
+__global__ void __launch_bounds__(kRedBlock) k_gershgorin(const double* F, int n, double* red) {
+  __shared__ double lo[kRedBlock], hi[kRedBlock];
+  double l = 1e300, h = -1e300;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double r = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) r += fabs(F[(size_t)j * n + i]);
+    const double d = F[(size_t)i * n + i];
+    l = fmin(l, d - r);
+    h = fmax(h, d + r);
+  }
+  lo[threadIdx.x] = l;
+  hi[threadIdx.x] = h;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + o]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { red[0] = lo[0]; red[1] = hi[0]; }
+}
+
+// X = (e_max I - F) / (e_max - e_min): the spectrum mapped into [0, 1],
+// lowest eigenvalues of F nearest 1
+__global__ void k_sp2_init(double* X, const double* F, int n, const double* red) {
+  const double emin = red[0], emax = red[1], s = 1.0 / (emax - emin);
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    X[e] = ((i == j ? emax : 0.0) - F[e]) * s;
+  }
+}
+
+// One workgroup: red[2] = tr X, red[3] = sum_ij X_ij^2 = tr X^2 (X symmetric)
+__global__ void __launch_bounds__(kRedBlock) k_sp2_trace(const double* X, int n, double* red) {
+  __shared__ double t1[kRedBlock], t2[kRedBlock];
+  const size_t nn = (size_t)n * n;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += X[(size_t)i * n + i];
+  for (size_t e = threadIdx.x; e < nn; e += blockDim.x) b += X[e] * X[e];
+  t1[threadIdx.x] = a;
+  t2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      t1[threadIdx.x] += t1[threadIdx.x + o];
+      t2[threadIdx.x] += t2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { red[2] = t1[0]; red[3] = t2[0]; }
+}
+
+// SP2 step: X <- X^2 if that brings the trace nearer nOcc, else 2X - X^2
+__global__ void k_sp2_update(double* X, const double* X2, size_t nn, const double* red, int nocc) {
+  const double tr = red[2], tr2 = red[3];
+  const bool sq = fabs(tr2 - nocc) < fabs(2.0 * tr - tr2 - nocc);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x)
+    X[e] = sq ? X2[e] : 2.0 * X[e] - X2[e];
 }
 
 int read_info() {
@@ -140,6 +207,71 @@ int sqla_density(int n, int nOcc, const double* C, double* P) {
                     g_w.A, n, &zero, g_w.P, n) != rocblas_status_success)
     return -4;
   if (hipMemcpy(P, g_w.P, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  return 0;
+}
+
+// Density P = sum_{k < nOcc} C_k C_k^T of the lowest nOcc solutions of
+// F C = S C e, without the eigenvectors: second-order spectral projection
+// (SP2, Niklasson, Phys. Rev. B 66, 155115 (2002)) of F' = X^T F X in the
+// orthonormal basis, then P = X P' X^T.  The iteration maps the spectrum
+// into [0, 1] by the Gershgorin bounds and squares towards the projector
+// (X^2 or 2X - X^2, whichever brings the trace nearer nOcc), one DGEMM per
+// step.  Returns 1 (P untouched) when the projector does not settle within
+// the step limit: no gap at nOcc (degenerate frontier orbitals) or a very
+// small one; the caller then uses the eigenvectors.
+int sqla_density_sp2(int n, int nOcc, const double* F, const double* S, double* P) {
+  if (n < 1 || nOcc < 0 || nOcc > n) return -1;
+  const size_t nn = (size_t)n * n;
+  if (nOcc == 0 || nOcc == n) {
+    if (nOcc == 0) { std::memset(P, 0, nn * sizeof(double)); return 0; }
+    return 1;  // full occupation: P = S^-1, left to the eigenvector path
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (ensure(n)) return -2;
+  if (int rc = set_metric(n, S)) return rc;
+  const double one = 1.0, zero = 0.0;
+  if (hipMemcpy(g_w.A, F, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
+  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.A, n, g_w.X, n,
+                    &zero, g_w.P, n) != rocblas_status_success ||
+      rocblas_dgemm(g_w.h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
+                    &zero, g_w.A, n) != rocblas_status_success)
+    return -4;
+  const unsigned grid = (unsigned)std::min<size_t>((nn + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(kRedBlock), 0, 0, g_w.A, n, g_w.red);
+  hipLaunchKernelGGL(k_sp2_init, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.A, n, g_w.red);
+  if (hipGetLastError() != hipSuccess) return -3;
+  // P holds the iterate, B its square.  The idempotency error tr X - tr X^2
+  // is read every 8 steps; once below 1e-11 four more steps take it to
+  // rounding level (each step squares it).
+  constexpr int kMaxSteps = 96, kCheck = 8;
+  int extra = -1;
+  double red[4] = {0, 0, 0, 0};
+  for (int it = 0; it < kMaxSteps; ++it) {
+    hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
+    if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.P, n, g_w.P, n,
+                      &zero, g_w.B, n) != rocblas_status_success)
+      return -4;
+    hipLaunchKernelGGL(k_sp2_update, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.B, nn, g_w.red, nOcc);
+    if (hipGetLastError() != hipSuccess) return -3;
+    if (extra > 0 && --extra == 0) break;
+    if (extra < 0 && (it + 1) % kCheck == 0) {
+      hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
+      if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+      if (!(red[2] == red[2]) || !(red[3] == red[3])) return 1;  // NaN
+      if (std::fabs(red[2] - red[3]) < 1e-11) extra = 4;
+    }
+  }
+  if (extra != 0) return 1;
+  hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
+  if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (!(std::fabs(red[2] - red[3]) < 1e-11) || !(std::fabs(red[2] - nOcc) < 1e-8)) return 1;
+  // P_AO = X P' X^T
+  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
+                    &zero, g_w.B, n) != rocblas_status_success ||
+      rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, g_w.B, n, g_w.X, n,
+                    &zero, g_w.A, n) != rocblas_status_success)
+    return -4;
+  if (hipMemcpy(P, g_w.A, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   return 0;
 }
 

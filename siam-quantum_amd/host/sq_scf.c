@@ -216,6 +216,25 @@ static void eigen_or_die(int n, int nOcc, const double *F, const double *S, doub
 	}
 }
 
+/* The density of the lowest nOcc solutions of F C = S C e.  With use_proj it
+ * comes straight from F (spectral projection, no orbitals; e and C are not
+ * updated); otherwise, or when the projection does not settle (no gap at
+ * nOcc), from the orbitals as in the reference (src/uhf.c:1703-1710). */
+static void density_from_fock(int n, int nOcc, const double *F, const double *S, double *e, double *C,
+                              double *P, int use_proj)
+{
+	if (use_proj) {
+		int rc = sqla_density_sp2(n, nOcc, F, S, P);
+		if (rc == 0) return;
+		if (rc < 0) {
+			printf("uhf: Error - device density projection failed (%d)\n", rc);
+			exit(EXIT_FAILURE);
+		}
+	}
+	eigen_or_die(n, nOcc, F, S, e, C);
+	density_or_die(n, nOcc, C, P);
+}
+
 double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, int nEB, double *CA,
            double *CB, double *eA, double *eB, struct option_t *opt)
 {
@@ -319,23 +338,24 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			FB[e] = H[e] + JT[e] + KB[e];
 		}
 		tp[5] += now_s() - t0;
-		/* the device eigen-solver returns S-orthonormal orbitals, so the
-		 * reference's re-normalisation (src/uhf.c:1705-1706) is a no-op here.
-		 * Restricted: KB = KA, so FB = FA and the beta solve would repeat
-		 * the alpha one exactly. */
+		/* The new densities come straight from FA, FB by spectral
+		 * projection; the orbitals and orbital energies are solved once,
+		 * after the loop, from the last FA, FB (or every iteration when
+		 * -SAVECHECKALL writes them).  The device eigen-solver returns
+		 * S-orthonormal orbitals, so the reference's re-normalisation
+		 * (src/uhf.c:1705-1706) is a no-op here.  Restricted: KB = KA, so
+		 * FB = FA and the beta solve would repeat the alpha one exactly. */
 		t0 = now_s();
-		eigen_or_die(n, nEA, FA, S, eA, CA);
+		density_from_fock(n, nEA, FA, S, eA, CA, PA, !opt->saveCheckAll);
 		if (opt->restricted && nEA == nEB) {
-			memcpy(CB, CA, nn * sizeof(double));
-			memcpy(eB, eA, (size_t)n * sizeof(double));
+			memcpy(PB, PA, nn * sizeof(double));
+			if (opt->saveCheckAll) {
+				memcpy(CB, CA, nn * sizeof(double));
+				memcpy(eB, eA, (size_t)n * sizeof(double));
+			}
 		} else {
-			eigen_or_die(n, nEB, FB, S, eB, CB);
+			density_from_fock(n, nEB, FB, S, eB, CB, PB, !opt->saveCheckAll);
 		}
-		tp[3] += now_s() - t0;
-		t0 = now_s();
-		density_or_die(n, nEA, CA, PA);
-		if (opt->restricted && nEA == nEB) memcpy(PB, PA, nn * sizeof(double));
-		else density_or_die(n, nEB, CB, PB);
 		tp[4] += now_s() - t0;
 		t0 = now_s();
 		dE = Etot;
@@ -361,6 +381,17 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 			}
 		}
 	} while (notConverged);
+
+	/* orbitals and orbital energies of the last Fock matrices */
+	t0 = now_s();
+	eigen_or_die(n, nEA, FA, S, eA, CA);
+	if (opt->restricted && nEA == nEB) {
+		memcpy(CB, CA, nn * sizeof(double));
+		memcpy(eB, eA, (size_t)n * sizeof(double));
+	} else {
+		eigen_or_die(n, nEB, FB, S, eB, CB);
+	}
+	tp[3] += now_s() - t0;
 
 	/* density and checkpoint files (src/uhf.c:1843-1883) */
 	if (opt->saveDMatrix) sq_save_density(n, opt->DMatrixFile, PA, PB);

@@ -96,3 +96,35 @@ def test_checkpoint_and_density_restart(tmp_path, monkeypatch, capfd):
     print(f"E = {E0:.10f} ({n0} it)  CHECK {E1:.10f} ({n1} it)  LDMATRIX {E2:.10f} ({n2} it)")
     assert abs(E1 - E0) < 1e-9 and abs(E2 - E0) < 1e-9
     assert n1 < n0 and n2 < n0
+
+
+def test_density_projection_matches_orbitals():
+    """sqla_density_sp2 (density straight from F by spectral projection)
+    against the density of the device eigenvectors (sqla_gen_eigen +
+    sqla_density) for water 6-31G*; a Fock matrix with no gap at nOcc is
+    refused (return 1) so that uhf() falls back to the eigenvectors."""
+    L = lib()
+    PD = C.POINTER(C.c_double)
+    L.sqla_density_sp2.argtypes = [C.c_int, C.c_int, PD, PD, PD]
+    L.sqla_gen_eigen.argtypes = [C.c_int, PD, PD, PD, PD]
+    L.sqla_density.argtypes = [C.c_int, C.c_int, PD, PD]
+    L.GTO_overlap.argtypes = [C.c_int, C.c_int, C.POINTER(GTOBasis)]
+    L.GTO_overlap.restype = C.c_double
+    mol, bas = g2("h2o_q0m1", "631g_star")
+    n = bas.n
+    S = np.array([[L.GTO_overlap(i, j, bas.ptr) for j in range(n)] for i in range(n)])
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((n, n))
+    F = S @ np.diag(np.linspace(-20.0, 5.0, n)) @ S + 0.05 * (A + A.T)
+    for nocc in (1, 5, 12):
+        P1 = np.zeros((n, n))
+        assert L.sqla_density_sp2(n, nocc, dptr(F), dptr(S), dptr(P1)) == 0
+        e, Cm, P2 = np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
+        assert L.sqla_gen_eigen(n, dptr(F), dptr(S), dptr(e), dptr(Cm)) == 0
+        assert L.sqla_density(n, nocc, dptr(Cm), dptr(P2)) == 0
+        print(nocc, np.abs(P1 - P2).max())
+        assert np.abs(P1 - P2).max() < 1e-10
+        assert abs(np.trace(P1 @ S) - nocc) < 1e-9
+    P3 = np.full((n, n), 7.0)
+    assert L.sqla_density_sp2(n, 3, dptr(S.copy()), dptr(S), dptr(P3)) == 1  # F = S: every eigenvalue is 1
+    assert np.all(P3 == 7.0)
