@@ -572,7 +572,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
   c->plan_q = c->plan_q32 = 0;
   struct Row { int i, lo, hi; };
   std::vector<Row> rows, sel;
-  std::vector<int> order;
+  std::vector<int> order, pass, pass64;
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y)
       for (int fp = 0; fp < 2; ++fp) {
@@ -596,18 +596,24 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
             if (X == Y && b > a) continue;  // canonical: ket pair index <= bra pair index
             const auto &g1 = c->groups[X][a], &g2 = c->groups[Y][b];
             // kets of the group passing with bra Schwarz si: a prefix (sorted by sch)
-            auto count = [&](double si, double t) {
-              int lo = 0, hi = g2.count;
-              while (lo < hi) {
-                const int mid = (lo + hi) / 2;
-                if (si * ket[g2.begin + mid].sch >= t) lo = mid + 1; else hi = mid;
+            // kets passing with bra i: a prefix of the ket group (sorted by
+            // Schwarz, descending), whose length cannot grow along the bra
+            // group (also sorted descending), so one sweep per cut-off
+            auto sweep = [&](double t, std::vector<int>& out) {
+              out.resize(g1.count);
+              int p = g2.count;
+              for (int k = 0; k < g1.count; ++k) {
+                const double si = bra[g1.begin + k].sch;
+                while (p > 0 && !(si * ket[g2.begin + p - 1].sch >= t)) --p;
+                out[k] = p;
               }
-              return lo;
             };
+            sweep(thr, pass);
+            sweep(thr64c, pass64);
             rows.clear();
             long long acc = 0;
             for (int i : order) {
-              int cnt = count(bra[i].sch, thr), c64 = std::min(count(bra[i].sch, thr64c), cnt);
+              int cnt = pass[i - g1.begin], c64 = std::min(pass64[i - g1.begin], cnt);
               if (X == Y && a == b) {
                 const int lim = i - g1.begin + 1;
                 cnt = std::min(cnt, lim);
