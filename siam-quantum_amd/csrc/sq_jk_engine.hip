@@ -266,11 +266,68 @@ struct sqjk_ctx {
 
 namespace {
 
+// Device and pinned host blocks released by a context are kept for the next
+// one (an SCF creates and releases its context once per solve; hipFree and
+// hipHostFree synchronise the device and cost more than the work they end).
+// A request takes the smallest cached block of the same kind and device that
+// holds it, at most twice its size; blocks beyond 4 GiB in total are freed.
+struct Block { void* p; size_t bytes; int device; bool host; };
+std::mutex g_block_mu;
+std::vector<Block> g_blocks;                   // cached, free
+std::vector<Block> g_live;                     // handed out
+size_t g_block_bytes = 0;
+constexpr size_t kBlockCacheMax = size_t(4) << 30;
+
+void* block_get(size_t bytes, bool host) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_block_mu);
+  size_t best = g_blocks.size();
+  for (size_t i = 0; i < g_blocks.size(); ++i) {
+    const Block& b = g_blocks[i];
+    if (b.host == host && (host || b.device == dev) && b.bytes >= bytes && b.bytes <= 2 * bytes &&
+        (best == g_blocks.size() || b.bytes < g_blocks[best].bytes))
+      best = i;
+  }
+  if (best < g_blocks.size()) {
+    Block b = g_blocks[best];
+    g_blocks.erase(g_blocks.begin() + (long)best);
+    g_block_bytes -= b.bytes;
+    g_live.push_back(b);
+    return b.p;
+  }
+  void* p = nullptr;
+  const hipError_t e = host ? hipHostMalloc(&p, bytes, hipHostMallocDefault) : hipMalloc(&p, bytes);
+  if (e != hipSuccess) return nullptr;
+  g_live.push_back({p, bytes, dev, host});
+  return p;
+}
+
+void block_put(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_block_mu);
+  for (size_t i = 0; i < g_live.size(); ++i)
+    if (g_live[i].p == p) {
+      Block b = g_live[i];
+      g_live.erase(g_live.begin() + (long)i);
+      if (g_block_bytes + b.bytes <= kBlockCacheMax) {
+        g_blocks.push_back(b);
+        g_block_bytes += b.bytes;
+      } else if (b.host) {
+        (void)hipHostFree(p);
+      } else {
+        (void)hipFree(p);
+      }
+      return;
+    }
+}
+
 template <typename T>
 int dalloc(T** p, size_t count) {
   if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) return fail(SQJK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  void* v = block_get(count * sizeof(T), false);
+  if (!v) return fail(SQJK_ENOMEM, "device allocation failed");
+  *p = static_cast<T*>(v);
   return SQJK_OK;
 }
 
@@ -397,10 +454,8 @@ int upload_static(sqjk_ctx* c) {
   if ((rc = dalloc(&c->d_ref, (size_t)2 * c->nref))) return rc;
   double** pins[] = {&c->hp_A, &c->hp_B, &c->hp_J, &c->hp_KA, &c->hp_KB};
   for (double** h : pins)
-    if (hipHostMalloc((void**)h, nn * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-      *h = nullptr;
+    if (!(*h = static_cast<double*>(block_get(nn * sizeof(double), true))))
       return fail(SQJK_ENOMEM, "pinned host allocation failed");
-    }
   std::vector<int> ref2((size_t)2 * c->nref);
   for (int P = 0; P < c->nref; ++P) {
     ref2[2 * P] = c->ref[P].first;
@@ -417,7 +472,7 @@ int upload_static(sqjk_ctx* c) {
 
 int upload_pairs(sqjk_ctx* c) {
   for (int X = 0; X < kNPairClass; ++X) {
-    if (c->d_pairs[X]) { (void)hipFree(c->d_pairs[X]); c->d_pairs[X] = nullptr; }
+    if (c->d_pairs[X]) { block_put(c->d_pairs[X]); c->d_pairs[X] = nullptr; }
     int rc = dalloc(&c->d_pairs[X], c->pairs[X].size());
     if (rc) return rc;
     if (!c->pairs[X].empty())
@@ -671,7 +726,7 @@ int make_plan(sqjk_ctx* c, double thr, double thr64) {
   if (c->h_rbra.size() > (size_t)INT32_MAX) return fail(SQJK_ENOMEM, "work plan too large");
   auto grow = [&](auto** d, size_t& cap, size_t need) -> int {
     if (cap >= need && *d) return SQJK_OK;
-    if (*d) { (void)hipFree(*d); *d = nullptr; }
+    if (*d) { block_put(*d); *d = nullptr; }
     cap = std::max<size_t>(need, 1024) * 3 / 2;
     return dalloc(d, cap);
   };
@@ -682,7 +737,7 @@ int make_plan(sqjk_ctx* c, double thr, double thr64) {
   if (c->row_cap < nrow || !c->d_rbra) {
     int** rowbufs[] = {&c->d_rbra, &c->d_rlo, &c->d_rhi};
     for (int** d : rowbufs) {
-      if (*d) (void)hipFree(*d);
+      if (*d) block_put(*d);
       *d = nullptr;
     }
     c->row_cap = std::max<size_t>(nrow, 1024) * 3 / 2;
@@ -771,12 +826,12 @@ int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, 
   std::vector<void*> bufs;
   auto up = [&](const void* h, size_t bytes) -> void* {
     void* d = nullptr;
-    if (hipMalloc(&d, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+    if (!(d = block_get(std::max<size_t>(bytes, 8), false))) return nullptr;
     bufs.push_back(d);
     if (h && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return d;
   };
-  auto release = [&] { for (void* d : bufs) (void)hipFree(d); };
+  auto release = [&] { for (void* d : bufs) block_put(d); };
   const std::vector<double>& boys = boys_table();
   OneEArgs a{};
   a.sh = (const SubShell*)up(c.sub.data(), c.sub.size() * sizeof(SubShell));
@@ -1028,16 +1083,16 @@ void sqjk_destroy(sqjk_ctx* c) {
   void* ptrs[] = {c->d_sub, c->d_pp, c->d_boys, c->d_bscale, c->d_DJ, c->d_DA, c->d_DB, c->d_PM, c->d_AJ,
                   c->d_AKA, c->d_AKB, c->d_J,   c->d_KA,    c->d_KB, c->d_sch, c->d_seg, c->d_tile, c->d_rbra, c->d_rlo, c->d_rhi};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  if (c->d_ref) (void)hipFree(c->d_ref);
+    block_put(p);
+  block_put(c->d_ref);
   double* pins[] = {c->hp_A, c->hp_B, c->hp_J, c->hp_KA, c->hp_KB};
   for (double* h : pins)
-    if (h) (void)hipHostFree(h);
+    block_put(h);
   for (int X = 0; X < kNPairClass; ++X) {
-    if (c->d_pairs[X]) (void)hipFree(c->d_pairs[X]);
-    if (c->d_Jp[X]) (void)hipFree(c->d_Jp[X]);
-    if (c->d_SA[X]) (void)hipFree(c->d_SA[X]);
-    if (c->d_SB[X]) (void)hipFree(c->d_SB[X]);
+    block_put(c->d_pairs[X]);
+    block_put(c->d_Jp[X]);
+    block_put(c->d_SA[X]);
+    block_put(c->d_SB[X]);
   }
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
