@@ -548,7 +548,7 @@ constexpr int class_block() {
 
 template <int LA, int LB, int LC, int LD>
 constexpr size_t class_lds_bytes() {
-  return sizeof(double) * kBoysNT * kBoysLdsW;
+  return sizeof(double) * boys_rows<LA + LB + LC + LD>() * kBoysLdsW;
 }
 
 }  // namespace sqjk

@@ -74,11 +74,24 @@ constexpr int kBoysCols = kBoysNM + 1;
 constexpr int kBoysTaylor = 9;  // terms of the Taylor expansion
 constexpr int kBoysLdsW = kBoysTaylor + 1;  // LDS row: F_L .. F_{L+8}, exp(-T_k)
 
+// Large-T branch of boys<L>: from T >= kBoysTCut[L] - 0.05 on, F_0 =
+// sqrt(pi/T)/2 and the upward recursion without the exp(-T) term are within
+// 2e-17 relative of F_m for every m <= L (checked against mpmath in steps of
+// 0.5), so a class of total angular momentum L needs table rows up to
+// kBoysTCut[L] only.
+constexpr double kBoysTCut[9] = {36.5, 40.5, 44.0, 47.0, 50.0, 52.5, 55.0, 57.5, 60.0};
+template <int L>
+constexpr int boys_rows() {
+  static_assert(L >= 0 && L <= 8, "Boys table cut-offs cover L <= 8");
+  return (int)(kBoysTCut[L] / kBoysStep + 0.5) + 1;
+}
+static_assert(boys_rows<8>() <= kBoysNT, "table shorter than the largest cut-off");
+
 // LDS copy for a class of total angular momentum L: per grid point the
 // Taylor coefficients F_L..F_{L+8} and exp(-T_k).
 template <int L>
 __device__ __forceinline__ void boys_load_lds(double* lds, const double* __restrict__ tab) {
-  for (int e = threadIdx.x + threadIdx.y * blockDim.x; e < kBoysNT * kBoysLdsW;
+  for (int e = threadIdx.x + threadIdx.y * blockDim.x; e < boys_rows<L>() * kBoysLdsW;
        e += blockDim.x * blockDim.y) {
     const int k = e / kBoysLdsW, j = e - k * kBoysLdsW;
     lds[e] = tab[k * kBoysCols + (j < kBoysTaylor ? L + j : kBoysNM)];
@@ -102,19 +115,18 @@ __host__ __device__ __forceinline__ float rsqrt_nr(float s) { return 1.0f / sqrt
 
 
 // F_0..F_L at T, evaluated in precision R (the table itself is FP64).
-// T < 59.95: F_L by Taylor expansion about the nearest grid point T_k,
+// T < kBoysTCut[L] - 0.05: F_L by Taylor expansion about the nearest grid point T_k,
 // exp(-T) = exp(-T_k) exp(T_k - T) with the second factor by its Taylor
 // series (|T_k - T| <= 0.05), then downward recursion.  Larger T: F_0 =
 // sqrt(pi/T)/2 from the refined reciprocal square root (erfc(sqrt(T)) <
 // 1e-27), and the upward recursion F_{m+1} = (2m+1) F_m / (2T) without the
-// exp(-T) term, which changes F_m (m <= 8) by less than 2e-17 relative for
-// T >= 59.95, so no exponential is evaluated on this branch.  The table reaches T = 60 so that
-// the branch needs none.  FP32 evaluation needs fewer Taylor terms for the
-// same relative accuracy.
+// exp(-T) term, which changes F_m (m <= L) by less than 2e-17 relative
+// there, so no exponential is evaluated on this branch.  FP32 evaluation
+// needs fewer Taylor terms for the same relative accuracy.
 template <int L, typename R = double>
 __host__ __device__ __forceinline__ void boys(R T, R (&F)[L + 1], const double* lds) {
   constexpr int NT = sizeof(R) == 4 ? 5 : kBoysTaylor;  // 0.05^5/5! ~ 2.6e-9
-  if (T < R(kBoysTMax - 0.5 * kBoysStep)) {
+  if (T < R(kBoysTCut[L] - 0.5 * kBoysStep)) {
     const int k = (int)(T * R(1.0 / kBoysStep) + R(0.5));
     const R d = R(k * kBoysStep) - T;  // expansion in (T_k - T)
     const double* row = lds + k * kBoysLdsW;

@@ -118,7 +118,8 @@ template <int LA, int LB>
 int launch_one(const OneEArgs& a, hipStream_t st) {
   if (a.np == 0) return 0;
   const int grid = (a.np + 255) / 256;
-  hipLaunchKernelGGL((k_one_e<LA, LB>), dim3(grid), dim3(256), sizeof(double) * kBoysNT * kBoysLdsW, st, a);
+  hipLaunchKernelGGL((k_one_e<LA, LB>), dim3(grid), dim3(256), sizeof(double) * boys_rows<LA + LB>() * kBoysLdsW, st,
+                     a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
