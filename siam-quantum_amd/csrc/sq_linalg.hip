@@ -105,13 +105,14 @@ __global__ void k_sp2_init(double* X, const double* F, int n, const double* red)
   }
 }
 
-// One workgroup: red[2] = tr X, red[3] = sum_ij X_ij^2 = tr X^2 (X symmetric)
-__global__ void __launch_bounds__(kRedBlock) k_sp2_trace(const double* X, int n, double* red) {
+// One workgroup: red[2] = tr X, red[3] = tr X2 (X2 = X X from the DGEMM)
+__global__ void __launch_bounds__(kRedBlock) k_sp2_trace(const double* X, const double* X2, int n, double* red) {
   __shared__ double t1[kRedBlock], t2[kRedBlock];
-  const size_t nn = (size_t)n * n;
   double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) a += X[(size_t)i * n + i];
-  for (size_t e = threadIdx.x; e < nn; e += blockDim.x) b += X[e] * X[e];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a += X[(size_t)i * n + i];
+    b += X2[(size_t)i * n + i];
+  }
   t1[threadIdx.x] = a;
   t2[threadIdx.x] = b;
   __syncthreads();
@@ -241,28 +242,33 @@ int sqla_density_sp2(int n, int nOcc, const double* F, const double* S, double* 
   hipLaunchKernelGGL(k_sp2_init, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.A, n, g_w.red);
   if (hipGetLastError() != hipSuccess) return -3;
   // P holds the iterate, B its square.  The idempotency error tr X - tr X^2
-  // is read every 8 steps; once below 1e-11 four more steps take it to
-  // rounding level (each step squares it).
+  // of the iterate entering a step is read every 8 steps; once below 1e-11
+  // four more steps take it to rounding level (each step squares it).
   constexpr int kMaxSteps = 96, kCheck = 8;
   int extra = -1;
   double red[4] = {0, 0, 0, 0};
-  for (int it = 0; it < kMaxSteps; ++it) {
-    hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
+  auto square = [&]() -> int {
     if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.P, n, g_w.P, n,
                       &zero, g_w.B, n) != rocblas_status_success)
       return -4;
+    hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, g_w.B, n, g_w.red);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+  };
+  for (int it = 0; it < kMaxSteps; ++it) {
+    if (int rc = square()) return rc;
+    if (extra < 0 && (it + 1) % kCheck == 0) {
+      if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+      if (!(red[2] == red[2]) || !(red[3] == red[3])) return 1;  // NaN
+      // a spectrum of rounding-level width has no meaningful lowest nOcc
+      if (!(red[1] - red[0] > 1e-10 * std::max(1.0, std::fabs(red[1])))) return 1;
+      if (std::fabs(red[2] - red[3]) < 1e-11) extra = 4;
+    }
     hipLaunchKernelGGL(k_sp2_update, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.B, nn, g_w.red, nOcc);
     if (hipGetLastError() != hipSuccess) return -3;
     if (extra > 0 && --extra == 0) break;
-    if (extra < 0 && (it + 1) % kCheck == 0) {
-      hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
-      if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
-      if (!(red[2] == red[2]) || !(red[3] == red[3])) return 1;  // NaN
-      if (std::fabs(red[2] - red[3]) < 1e-11) extra = 4;
-    }
   }
   if (extra != 0) return 1;
-  hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, n, g_w.red);
+  if (int rc = square()) return rc;
   if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
   if (!(std::fabs(red[2] - red[3]) < 1e-11) || !(std::fabs(red[2] - nOcc) < 1e-8)) return 1;
   // P_AO = X P' X^T
