@@ -56,9 +56,12 @@ struct Segment {
 
 // Row r: bra pair rbra[r] (class-list index) with the ket pairs at positions
 // [rlo[r], rhi[r]) of its segment's ket group.  A tile covers rows [r0, r1)
-// (one bra shell a) and ket positions [k0, k0 + 64).
+// (one bra shell a), ket positions [k0, k0 + 64) and the ket primitive pairs
+// [v0, v1): the primitive sum of a highly contracted segment is split over
+// several tiles (each digests its partial integrals; J and K are linear in
+// them), so that no tile is much longer than the others.
 struct Tile {
-  int seg, r0, r1, k0;
+  int seg, r0, r1, k0, v0, v1;
 };
 
 enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
@@ -200,14 +203,15 @@ struct RunAcc {
   }
 };
 
-// Contracted quartet (ib, ik), ket components [CD0, CD1).  R: precision of
+// Contracted quartet (ib, ik) over the bra primitive pairs [0, nppb) and the
+// ket primitive pairs [v0, v1), ket components [CD0, CD1).  R: precision of
 // the integral evaluation (double, or float for the mixed-precision range);
 // digestion and accumulation are FP64.  All lanes of a wavefront call this
 // together; `valid` gates the lane's contributions, so cross-lane reductions
 // see every lane.
 // RUN: J_cd and the a-side exchange go to `run` instead of memory.
 template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE, typename R = double, bool RUN = false>
-__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, int nppb, int nppk,
+__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, int nppb, int v0, int v1,
                                                  bool valid, const double* lds,
                                                  RunAcc<LA, LC, LD, CD0, CD1>* run = nullptr) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -223,7 +227,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
 
   // ket primitive outer (per-lane data, loaded once), bra primitive inner
   // (the same record for every lane of a bra-uniform wavefront)
-  for (int v = 0; v < nppk; ++v) {
+  for (int v = v0; v < v1; ++v) {
     const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
     const R h2q = R(Q.hp);
     Herm1<LC, LD, R> Fx, Fy, Fz;
@@ -504,7 +508,7 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
       const long long stride = (long long)gridDim.x * blockDim.x;
       for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
         const int npp = A.bra[t].npp;
-        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, npp, true, lds);
+        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, 0, npp, true, lds);
       }
     } else {
       const int lane = threadIdx.x & 63;
@@ -526,7 +530,7 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
           any = any || in;
           const bool valid = in && density_screen_pass(A, uload(A.bra + ib), A.ket[ik]);
           if (!__any(valid)) continue;  // the whole wavefront screened out: skip the row (uniform)
-          quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, sg.nppk, valid, lds, &run);
+          quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, tl.v0, tl.v1, valid, lds, &run);
         }
         if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, uload(A.sh + (uload(A.bra + (uload(A.rbra + tl.r0))).a)).bf_off, sg.ket0 + j);
       }

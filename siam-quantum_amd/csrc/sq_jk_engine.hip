@@ -588,6 +588,12 @@ void host_parallel(int maxt, F&& fn) {
   for (auto& th : pool) th.join();
 }
 
+// Primitive quartets per quartet above which a tile's ket primitive pairs
+// are split over several tiles: the (ss|ss) blocks of two six-primitive core
+// shells (1296 primitive quartets per lane) otherwise set the length of a
+// whole launch for small molecules.
+constexpr int kSplitPrims = 128;
+
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
 // in registers across both rows and halve their atomics; that pays where the
@@ -702,7 +708,12 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                     c->h_rlo.push_back(sel[u].lo);
                     c->h_rhi.push_back(sel[u].hi);
                   }
-                  c->h_tile.push_back({seg, r0, (int)c->h_rbra.size(), k0});
+                  // a segment of more than kSplitPrims primitive quartets per
+                  // quartet is cut along its ket primitive pairs
+                  const int nch = std::min(g2.npp, (g1.npp * g2.npp + kSplitPrims - 1) / kSplitPrims);
+                  for (int ch = 0; ch < nch; ++ch)
+                    c->h_tile.push_back({seg, r0, (int)c->h_rbra.size(), k0, g2.npp * ch / nch,
+                                         g2.npp * (ch + 1) / nch});
                 }
               }
               r = e;
