@@ -588,11 +588,17 @@ void host_parallel(int maxt, F&& fn) {
   for (auto& th : pool) th.join();
 }
 
-// Primitive quartets per quartet above which a tile's ket primitive pairs
-// are split over several tiles: the (ss|ss) blocks of two six-primitive core
-// shells (1296 primitive quartets per lane) otherwise set the length of a
-// whole launch for small molecules.
-constexpr int kSplitPrims = 128;
+// Primitive quartets per lane and tile above which a tile's ket primitive
+// pairs are split over several tiles, for a class of nint integrals per
+// quartet: about 1152 integral-primitive products, between 8 (32 for classes
+// of more than 81 integrals, whose digestion is dearer than their short
+// primitive loops) and 128 primitive quartets.  The (ss|ss) blocks of two
+// six-primitive core shells (1296 primitive quartets per lane) or the (pp|pp)
+// blocks of contracted p shells otherwise set the length of a whole launch
+// for small molecules.
+int split_prims(int nint) {
+  return std::max(nint > 81 ? 32 : 8, std::min(128, 1152 / std::max(1, nint)));
+}
 
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
@@ -710,7 +716,8 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                   }
                   // a segment of more than kSplitPrims primitive quartets per
                   // quartet is cut along its ket primitive pairs
-                  const int nch = std::min(g2.npp, (g1.npp * g2.npp + kSplitPrims - 1) / kSplitPrims);
+                  const int sp = split_prims(nint);
+                  const int nch = std::min(g2.npp, (g1.npp * g2.npp + sp - 1) / sp);
                   for (int ch = 0; ch < nch; ++ch)
                     c->h_tile.push_back({seg, r0, (int)c->h_rbra.size(), k0, g2.npp * ch / nch,
                                          g2.npp * (ch + 1) / nch});
