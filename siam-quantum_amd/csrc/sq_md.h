@@ -111,7 +111,14 @@ __host__ __device__ __forceinline__ double rsqrt_nr(double s) {
   return 1.0 / sqrt(s);
 #endif
 }
-__host__ __device__ __forceinline__ float rsqrt_nr(float s) { return 1.0f / sqrtf(s); }
+__host__ __device__ __forceinline__ float rsqrt_nr(float s) {
+#ifdef __HIP_DEVICE_COMPILE__
+  float y = __builtin_amdgcn_rsqf(s);  // hardware estimate, then one Newton step
+  return y * (1.5f - 0.5f * s * y * y);
+#else
+  return 1.0f / sqrtf(s);
+#endif
+}
 
 
 // F_0..F_L at T, evaluated in precision R (the table itself is FP64).
