@@ -590,14 +590,14 @@ void host_parallel(int maxt, F&& fn) {
 
 // Primitive quartets per lane and tile above which a tile's ket primitive
 // pairs are split over several tiles, for a class of nint integrals per
-// quartet: about 1152 integral-primitive products, between 8 (32 for classes
+// quartet: about 576 integral-primitive products, between 8 (32 for classes
 // of more than 81 integrals, whose digestion is dearer than their short
-// primitive loops) and 128 primitive quartets.  The (ss|ss) blocks of two
+// primitive loops) and 64 primitive quartets.  The (ss|ss) blocks of two
 // six-primitive core shells (1296 primitive quartets per lane) or the (pp|pp)
 // blocks of contracted p shells otherwise set the length of a whole launch
 // for small molecules.
 int split_prims(int nint) {
-  return std::max(nint > 81 ? 32 : 8, std::min(128, 1152 / std::max(1, nint)));
+  return std::max(nint > 81 ? 32 : 8, std::min(64, 576 / std::max(1, nint)));
 }
 
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
