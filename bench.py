@@ -1,20 +1,30 @@
 #!/usr/bin/env python3
 """J/K builds per second of the MI355X two-electron engine.
 
-Workload (one "step" = one full J/K build, inputs resident on the host as in
-the reference's uhf_getGMatrix call): seeded water cluster (H2O)32
-(seed 3201, rigid monomers, O-O >= 2.7 A, 1 g/cm^3), RHF/6-31G* with
-Cartesian d, 608 basis functions; Schwarz and density-weighted screening at
-1e-12 (the reference's rules); seeded random symmetric density; FP64.
+Headline workload (one "step" = one full J/K build): the C60 truncated
+icosahedron (C-C 1.45/1.40 A), RHF/6-31G* with Cartesian d, 900 basis
+functions, Schwarz and density-weighted screening at 1e-10 (the reference's
+rules), FP64, on the density of a converged SCF of the same molecule
+(computed first by the uhf() drop-in on the device).  Inputs are host
+arrays, as in the reference's uhf_getGMatrix call, so `value` includes the
+density upload and the J/K download.
 
 Prints one JSON line.  `roofline` is the FP64 vector-ALU roofline of the
-integral kernels (modelled flops per build / HIP-event kernel time), and
-`cpu_baseline` times the reference's own code (oracle/_ref, built from the
-reference sources) on a bounded slice of the same build on one host core.
+dominant class kernel: the useful work of one launch (surviving primitive
+quartets x modelled flops per primitive quartet + surviving quartets x
+digestion flops) over its HIP-event time on the engine's stream;
+`traffic` is the measured HBM bytes of that launch from the committed
+rocprofv3 counter summary.  `cpu_baseline` times the reference's own code
+(oracle/_ref, built from the reference sources) on a bounded slice of the
+same build on one host core.
+Further keys: the (H2O)32 build, the incremental (Delta-D) build, the mixed
+FP32/FP64 build, SCF solves/s with p50/p99 latency.
 """
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,58 +35,71 @@ sys.path.insert(0, os.path.join(ROOT, "siam-quantum_amd"))
 
 METRIC = "J/K builds/s & SCF solves/s (p50/p99 lat), mixed vs FP64 GPU at |ΔE|≤1µEh"
 PEAK_FP64_TFLOPS = 78.6
-WORKLOADS = {
-    "w32": (32, 3201),
-    "w16": (16, 1601),
-}
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-def make_inputs(name):
+def molecule(name):
     from sqamd import inputs
-    nmol, seed = WORKLOADS[name]
-    Z, X = inputs.water_cluster(nmol, seed)
+    if name == "c60":
+        Z, X = inputs.c60()
+        desc = "C60 truncated icosahedron (C-C 1.45/1.40 A)"
+    else:
+        nm, seed = {"w16": (16, 1601), "w32": (32, 3201)}[name]
+        Z, X = inputs.water_cluster(nm, seed)
+        desc = f"(H2O){nm} cluster, seed {seed}"
     mol = inputs.mol_from_angstrom(Z, X)
-    bas = inputs.make_basis(mol, "631g_star")
-    rng = np.random.default_rng(seed)
-    A = rng.standard_normal((bas.n, bas.n)) * 0.01
-    return mol, bas, A + A.T
+    return mol, inputs.make_basis(mol, "631g_star"), desc
 
 
-def scf_solves(name, nsolve, mixed_thr):
-    """Repeated RHF SCF solves through the uhf() drop-in (reference defaults:
-    3-step cut-offs, DIIS4, SCFConv 1e-8): each solve builds its per-molecule
-    state, runs the SCF on the device and releases the state, as uhf() does.
-    Returns FP64 and mixed-precision solves/s with p50/p99 latency and the
-    energy difference."""
-    import ctypes as C
-    from sqamd import inputs, lib, default_option, dptr, GTOBasis, Molecule, Option
-    nmol, seed = WORKLOADS[name]
-    Z, X = inputs.water_cluster(nmol, seed)
-    mol = inputs.mol_from_angstrom(Z, X)
-    bas = inputs.make_basis(mol, "631g_star")
+def uhf_fn():
+    from sqamd import lib, GTOBasis, Molecule, Option
     L = lib()
     L.uhf.argtypes = [C.c_int, C.POINTER(GTOBasis), C.POINTER(Molecule), C.c_int, C.c_int] + \
         [C.POINTER(C.c_double)] * 4 + [C.POINTER(Option)]
     L.uhf.restype = C.c_double
     L.sqjk_set_dropin_mixed.argtypes = [C.c_double]
+    return L
+
+
+def scf_once(mol, bas, **kw):
+    """RHF through the uhf() drop-in; returns (E, density P_alpha, seconds)."""
+    from sqamd import default_option, dptr
+    L = uhf_fn()
     n, ne = bas.n, mol.n_electrons
-    out = {"workload": f"(H2O){nmol} RHF/6-31G* SCF, {n} basis functions, seed {seed}, "
-                       f"reference defaults with SCFConv 1e-8", "solves": nsolve}
+    CA, CB, eA, eB = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+    opt = default_option(**kw)
+    t0 = time.perf_counter()
+    E = L.uhf(n, bas.ptr, mol.ptr, ne // 2, ne // 2, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt))
+    dt = time.perf_counter() - t0
+    occ = CA[:ne // 2]
+    return E, occ.T @ occ, dt
+
+
+def scf_solves(name, nsolve, mixed_thr):
+    """Repeated RHF SCF solves through the uhf() drop-in (reference defaults:
+    3-step cut-offs, DIIS4, SCFConv 1e-8).  The first (cold) solve is
+    reported apart; p50/p99 are over the warm ones."""
+    from sqamd import default_option, dptr
+    mol, bas, desc = molecule(name)
+    L = uhf_fn()
+    n, ne = bas.n, mol.n_electrons
+    out = {"workload": f"{desc} RHF/6-31G* SCF, {n} basis functions, reference defaults with SCFConv 1e-8",
+           "warm_solves": nsolve}
     for label, thr in (("fp64", 0.0), ("mixed", mixed_thr)):
         if label == "mixed" and thr <= 0:
             continue
         L.sqjk_set_dropin_mixed(thr)
         lat, E = [], None
-        for _ in range(nsolve):
+        for _ in range(nsolve + 1):
             CA, CB, eA, eB = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
             opt = default_option(SCFConv=1e-8)
             t0 = time.perf_counter()
             E = L.uhf(n, bas.ptr, mol.ptr, ne // 2, ne // 2, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt))
             lat.append(time.perf_counter() - t0)
-        lat = np.array(lat)
-        out[label] = {"value": len(lat) / lat.sum(), "unit": "SCF solves/s",
-                      "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-                      "energy": E}
+        cold, warm = lat[0], np.array(lat[1:])
+        out[label] = {"value": len(warm) / warm.sum(), "unit": "SCF solves/s",
+                      "p50_ms": float(np.percentile(warm, 50) * 1e3), "p99_ms": float(np.percentile(warm, 99) * 1e3),
+                      "cold_ms": cold * 1e3, "energy": E}
         if label == "mixed":
             out[label]["threshold"] = thr
             out[label]["abs_dE"] = abs(E - out["fp64"]["energy"])
@@ -84,10 +107,39 @@ def scf_solves(name, nsolve, mixed_thr):
     return out
 
 
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, calibrated against a plain
+    copy), or None when no summary names this kernel."""
+    path = os.path.join(PROFILES, "jk_c60_631gs_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+)\s+(\d+)", line)
+        if m and m.group(1) == kernel:
+            return int(m.group(2)) + int(m.group(3)), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def timed_builds(eng, P, cut, steps, warmup):
+    for _ in range(warmup):
+        eng.build(P, P, cut, cut, True)
+    kms, cms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.build(P, P, cut, cut, True)
+        kms.append(eng.stats()["kernel_ms"])
+        cms.append(eng.class_times())
+    return time.perf_counter() - t0, float(np.mean(kms)), np.mean(np.array(cms), axis=0)
+
+
+def class_label(lq):
+    return "(%d%d|%d%d)" % tuple(lq)
+
+
 def cpu_baseline(bas, P, cut, slices):
     """Reference code (oracle/_ref) on the PQ % slices == 0 part of the same
     build, one core; builds/s scaled by the number of slices."""
-    import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Reference, default_option
     from sqamd import dptr
@@ -102,9 +154,11 @@ def cpu_baseline(bas, P, cut, slices):
                                          dptr(J), dptr(K1), dptr(K2), C.byref(opt))
     dt = time.perf_counter() - t0
     ref.jk_reset(opt)
-    return {"value": 1.0 / (dt * slices), "unit": "builds/s", "cores": 1, "kind": "reference",
+    return {"value": 1.0 / (dt * slices), "unit": "J/K builds/s", "cores": 1, "kind": "reference",
+            "host_cores_visible": os.cpu_count(),
             "sample": f"reference GTO_JK_Matrix_Quartet_Parallel, childID 0 of nCPU={slices} "
-                      f"(bra shell pairs PQ % {slices} == 0) of the same build, {dt:.1f} s, x{slices}"}
+                      f"(bra shell pairs PQ % {slices} == 0) of the same C60 build, {dt:.1f} s on one core, "
+                      f"x{slices}"}
 
 
 def main():
@@ -112,14 +166,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="w32", choices=sorted(WORKLOADS))
-    ap.add_argument("--cutoff", type=float, default=1e-12)
-    ap.add_argument("--cpu-slices", type=int, default=16)
+    ap.add_argument("--cutoff", type=float, default=1e-10)
+    ap.add_argument("--cpu-slices", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-threshold", type=float, default=1e-4,
                     help="Schwarz product below which quartets run in FP32 for the mixed line (0: skip)")
-    ap.add_argument("--scf-solves", type=int, default=5, help="SCF solves per precision (0: skip)")
-    ap.add_argument("--scf-workload", default="w16", choices=sorted(WORKLOADS))
+    ap.add_argument("--scf-solves", type=int, default=100, help="warm SCF solves per precision (0: skip)")
+    ap.add_argument("--scf-workload", default="w16", choices=["w16", "w32"])
+    ap.add_argument("--no-extras", action="store_true", help="only the headline C60 build line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -140,31 +194,27 @@ def main():
         sys.exit(2)
 
     from sqamd import JKEngine, workmodel
-    mol, bas, P = make_inputs(a.workload)
+    mol, bas, desc = molecule("c60")
     n = bas.n
+    print(f"bench: C60 SCF for the step density (n={n})", file=sys.stderr, flush=True)
+    E_scf, P, t_scf = scf_once(mol, bas, SCFConv=1e-6)
+    print(f"bench: C60 E = {E_scf:.8f} Eh in {t_scf:.1f} s; timing builds", file=sys.stderr, flush=True)
     eng = JKEngine(bas.ptr, n)
     eng.set_profile(True)
-    for _ in range(a.warmup):
-        eng.build(P, P, a.cutoff, a.cutoff, True)
-    kms, cms = [], []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        eng.build(P, P, a.cutoff, a.cutoff, True)
-        kms.append(eng.stats()["kernel_ms"])
-        cms.append(eng.class_times())
-    wall = time.perf_counter() - t0
+    wall, kernel_ms, cls_ms = timed_builds(eng, P, a.cutoff, a.steps, a.warmup)
     st = eng.stats()
-    cls = eng.class_stats()
-    flops = workmodel.build_flops(cls)
-    kernel_s = float(np.mean(kms)) / 1e3
-    achieved = flops / kernel_s / 1e12
-    # dominant class kernel: HIP events on the engine's stream around its launches
-    cls_ms = np.mean(np.array(cms), axis=0)
+    plan = eng.class_stats()
+    work = eng.class_work()
+    # useful work: surviving primitive quartets and quartets of each class
+    cflops = [wp * workmodel.flops_per_prim_quartet(*lq) + wq * workmodel.digestion_flops(*lq)
+              for lq, wq, wp, _, _ in work]
     dom = int(np.argmax(cls_ms))
-    dlq, dnq, dnpq = cls[dom]
-    dflops = dnpq * workmodel.flops_per_prim_quartet(*dlq) + dnq * workmodel.digestion_flops(*dlq)
-    dach = dflops / (cls_ms[dom] / 1e3) / 1e12
-    ms = wall / a.steps * 1e3
+    dlq = work[dom][0]
+    dname = "k_class_wave<2, 2, 2, 2, 0, double>" if dlq == (2, 2, 2, 2) else \
+        "k_class<%d, %d, %d, %d, 0, double, %d>" % (dlq + (512 if workmodel.class_block(*dlq) == 512 else 256,))
+    dach = cflops[dom] / (cls_ms[dom] / 1e3) / 1e12
+    build_ach = sum(cflops) / (kernel_ms / 1e3) / 1e12
+    traffic, traffic_src = profile_traffic(dname)
     out = {
         "metric": METRIC,
         "value": a.steps / wall,
@@ -172,55 +222,81 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"(H2O){WORKLOADS[a.workload][0]} RHF/6-31G* full J/K build, "
-                               f"{n} basis functions, seed {WORKLOADS[a.workload][1]}, "
-                               f"Schwarz+density screening {a.cutoff:g}, FP64",
-                   "basis_functions": n, "quartets": st["quartets"],
-                   "integrals": workmodel.integrals(cls)},
-        "classes": {"(%d%d|%d%d)" % lq: {"quartets": nq, "prim_quartets": npq,
-                                         "gflop": (npq * workmodel.flops_per_prim_quartet(*lq) +
-                                                   nq * workmodel.digestion_flops(*lq)) / 1e9,
-                                         "kernel_ms": float(cls_ms[k])}
-                    for k, (lq, nq, npq) in enumerate(cls)},
-        "kernel_ms": kernel_s * 1e3,
-        "roofline": {"bound": "valu-fp64", "kernel": "k_class (%d%d|%d%d) FP64" % dlq,
+        "config": {"workload": f"{desc} RHF/6-31G* full J/K build, {n} basis functions, Schwarz+density "
+                               f"screening {a.cutoff:g}, converged-SCF density, FP64, host arrays in/out",
+                   "basis_functions": n, "quartets_screened": st["quartets"],
+                   "quartets_surviving": int(sum(w[1] for w in work)),
+                   "prim_quartets_surviving": int(sum(w[2] for w in work)),
+                   "integrals_surviving": int(sum(wq * workmodel.nint(*lq) for lq, wq, _, _, _ in work)),
+                   "scf_energy": E_scf, "scf_seconds": t_scf},
+        "kernel_ms": kernel_ms,
+        "classes": {class_label(lq): {"quartets_plan": plan[k][1], "quartets": wq, "prim_quartets": wp,
+                                      "gflop": cflops[k] / 1e9, "kernel_ms": float(cls_ms[k]),
+                                      "frac": cflops[k] / (cls_ms[k] / 1e3) / 1e12 / PEAK_FP64_TFLOPS
+                                      if cls_ms[k] > 0 else None}
+                    for k, (lq, wq, wp, _, _) in enumerate(work)},
+        "roofline": {"bound": "valu-fp64", "kernel": dname,
                      "achieved": dach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": dach / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel_ms": float(cls_ms[dom]), "flops_per_launch": dflops,
-                     "build": {"achieved": achieved, "frac": achieved / PEAK_FP64_TFLOPS,
-                               "flops_per_build": flops, "kernel_ms": kernel_s * 1e3}},
+                     "frac": dach / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_ms": float(cls_ms[dom]), "flops_per_launch": cflops[dom],
+                     "units_per_launch": {"quartets": work[dom][1], "prim_quartets": work[dom][2]},
+                     "build": {"achieved": build_ach, "frac": build_ach / PEAK_FP64_TFLOPS,
+                               "flops_per_build": sum(cflops), "kernel_ms": kernel_ms}},
     }
-    if a.mixed_threshold > 0:
-        eng.set_mixed(a.mixed_threshold)
-        eng.build(P, P, a.cutoff, a.cutoff, True)
-        J64 = eng.build(P, P, a.cutoff, a.cutoff, True)  # warm, then compare below
-        t1 = time.perf_counter()
-        mk = []
+    if not a.no_extras:
+        # incremental build: a late-iteration density change (1e-4 scale)
+        rng = np.random.default_rng(60)
+        A = rng.standard_normal((n, n)) * 1e-4
+        dP = A + A.T
+        timed_builds(eng, dP, a.cutoff, 1, 1)
+        t0 = time.perf_counter()
         for _ in range(a.steps):
-            Jm, Km, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
-            mk.append(eng.stats()["kernel_ms"])
-        wall_m = time.perf_counter() - t1
-        stm = eng.stats()
-        eng.set_mixed(0.0)
-        Jd, Kd, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
-        out["mixed"] = {"value": a.steps / wall_m, "unit": "J/K builds/s", "threshold": a.mixed_threshold,
-                        "kernel_ms": float(np.mean(mk)), "quartets_fp32": stm["quartets_fp32"],
-                        "max_abs_dJ": float(np.abs(Jm - Jd).max()), "max_abs_dK": float(np.abs(Km - Kd).max()),
-                        "speedup_vs_fp64": (a.steps / wall_m) / out["value"]}
+            eng.build(dP, dP, a.cutoff, a.cutoff, True)
+        out["delta_d"] = {"ms_per_build": (time.perf_counter() - t0) / a.steps * 1e3,
+                          "kernel_ms": eng.stats()["kernel_ms"],
+                          "quartets_surviving": int(sum(w[1] for w in eng.class_work())),
+                          "density": "P_conv-like change: seeded symmetric N(0, 1e-4)"}
+        if a.mixed_threshold > 0:
+            J64, K64, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
+            eng.set_mixed(a.mixed_threshold)
+            eng.build(P, P, a.cutoff, a.cutoff, True)
+            t1 = time.perf_counter()
+            mk = []
+            for _ in range(a.steps):
+                Jm, Km, _ = eng.build(P, P, a.cutoff, a.cutoff, True)
+                mk.append(eng.stats()["kernel_ms"])
+            wall_m = time.perf_counter() - t1
+            stm = eng.stats()
+            eng.set_mixed(0.0)
+            out["mixed"] = {"value": a.steps / wall_m, "unit": "J/K builds/s", "threshold": a.mixed_threshold,
+                            "kernel_ms": float(np.mean(mk)), "quartets_fp32": stm["quartets_fp32"],
+                            "max_abs_dJ": float(np.abs(Jm - J64).max()), "max_abs_dK": float(np.abs(Km - K64).max()),
+                            "speedup_vs_fp64": (a.steps / wall_m) / out["value"]}
     eng.close()
-    if a.scf_solves > 0:
-        out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)
+    if not a.no_extras:
+        mol32, bas32, desc32 = molecule("w32")
+        rng = np.random.default_rng(3201)
+        A = rng.standard_normal((bas32.n, bas32.n)) * 0.01
+        e32 = JKEngine(bas32.ptr, bas32.n)
+        w32, k32, _ = timed_builds(e32, A + A.T, 1e-12, a.steps, a.warmup)
+        e32.close()
+        out["w32"] = {"workload": f"{desc32} RHF/6-31G*, {bas32.n} functions, screening 1e-12, seeded density",
+                      "value": a.steps / w32, "unit": "J/K builds/s", "kernel_ms": k32}
+        if a.scf_solves > 0:
+            print("bench: SCF solves", file=sys.stderr, flush=True)
+            out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)
     if not a.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         try:
             out["cpu_baseline"] = cpu_baseline(bas, P, a.cutoff, a.cpu_slices)
         except FileNotFoundError as e:
-            out["cpu_baseline"] = {"value": None, "unit": "builds/s", "cores": 1, "kind": "reference",
+            out["cpu_baseline"] = {"value": None, "unit": "J/K builds/s", "cores": 1, "kind": "reference",
                                    "sample": f"not run: {e}"}
     print(json.dumps(out))
 

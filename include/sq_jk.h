@@ -85,6 +85,17 @@ int sqjk_last_stats(const sqjk_ctx *ctx, sqjk_stats *st);
  * output pointer may be NULL.  Returns the number of classes. */
 int sqjk_class_stats(const sqjk_ctx *ctx, int *lq, long long *quartets, long long *prim_quartets);
 
+/* Work of the last build that survives all screening, per angular class
+ * (same class order as sqjk_class_stats): contracted shell quartets that pass
+ * the density-weighted test, and their primitive quartets whose bare
+ * prefactor passes the 1e-15 cut, split by the precision they were
+ * evaluated in.  Counted on the device with the last build's density and
+ * cut-offs (one extra pass over the work plan; no integrals).  Any output
+ * pointer may be NULL.  Returns the number of classes, or an error code.
+ * No counterpart in the reference (measurement support). */
+int sqjk_class_work(sqjk_ctx *ctx, long long *quartets, long long *prim_quartets,
+                    long long *quartets_fp32, long long *prim_quartets_fp32);
+
 /* Per-class kernel timing.  With `on` set, every later sqjk_build records a
  * pair of HIP events around each class launch group on the context's stream
  * (all ket-component slices, FP64 and FP32 launches of one class) and
@@ -101,6 +112,14 @@ int sqjk_class_times(const sqjk_ctx *ctx, double *kernel_ms);
  * src/int.c:392-393); the electric-field term is left to the caller. */
 int sqjk_one_electron(int nBasis, const struct GTOBasis_t *gto, const struct Molecule_t *mol, int device,
                       double *T, double *V, double *S);
+
+/* Boys function F_0..F_L (L <= 8) at nT points T >= 0, evaluated on the
+ * device by the integral kernels' own code (LDS table with Taylor expansion
+ * and downward recursion below the per-L cut-off, sqrt(pi/T)/2 with upward
+ * recursion above it), in FP32 when fp32 is set.  F[i*(L+1) + m] = F_m(T_i).
+ * Same quantity as the reference's fgamma_set (src/fgamma.c:172-206); this
+ * entry exists so that the device evaluation can be checked on its own. */
+int sqjk_boys(int L, int nT, const double *T, int fp32, int device, double *F);
 
 /* Number of basis functions, reference shells and pure-l sub-shells. */
 int sqjk_dims(const sqjk_ctx *ctx, int *nBasis, int *nShell, int *nSubShell);

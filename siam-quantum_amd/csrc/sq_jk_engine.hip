@@ -210,6 +210,75 @@ __global__ void __launch_bounds__(kReduceKBlock) k_reduce_k(KStage R, const SubS
   }
 }
 
+// Work that survives the screening in one launch of a class: contracted
+// quartets that pass the density-weighted test, and their primitive
+// quartets whose bare prefactor passes the 1e-15 cut.  Same tile walk as
+// the class kernels (one wavefront per tile, lane = ket position); a
+// quartet is counted once, by the tile that holds the first ket primitive
+// pair of its segment (tiles of a split segment repeat the same quartets).
+// cnt[0] += quartets, cnt[1] += primitive quartets.
+__global__ void __launch_bounds__(256) k_count_work(ClassArgs A, unsigned long long* cnt) {
+  const int lane = threadIdx.x & 63;
+  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
+  double q = 0.0, p = 0.0;
+  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < A.total; w += wstride) {
+    const Tile tl = A.tile[w];
+    const Segment sg = A.seg[tl.seg];
+    const int j = tl.k0 + lane;
+    for (int r = tl.r0; r < tl.r1; ++r) {
+      if (j < A.rlo[r] || j >= A.rhi[r]) continue;
+      const int ib = A.rbra[r], ik = sg.ket0 + j;
+      const PairRec bp = A.bra[ib], kp = A.ket[ik];
+      if (!density_screen_pass(A, bp, kp)) continue;
+      if (tl.v0 == 0) q += 1.0;
+      for (int u = 0; u < sg.nppb; ++u) {
+        const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
+        for (int v = tl.v0; v < tl.v1; ++v) {
+          const PrimPair Q = A.pp[kp.pp_off + v * kp.pp_stride];
+          if (!(kTwoPi25 * P.eb * Q.eb / sqrt(P.p + Q.p) < kPrimCutoff)) p += 1.0;
+        }
+      }
+    }
+  }
+  q = wave_sum(q);
+  p = wave_sum(p);
+  if (lane == 0) {
+    if (q > 0.0) atomicAdd(&cnt[0], (unsigned long long)q);
+    if (p > 0.0) atomicAdd(&cnt[1], (unsigned long long)p);
+  }
+}
+
+// F_0..F_L at nT points by the class kernels' Boys code (boys<L, R> from
+// the LDS table), for checking it against independent values.
+template <int L, typename R>
+__global__ void __launch_bounds__(256) k_boys_eval(const double* __restrict__ tab, const double* __restrict__ T,
+                                                   double* F, int nT) {
+  extern __shared__ double lds[];
+  boys_load_lds<L>(lds, tab);
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nT; i += gridDim.x * blockDim.x) {
+    R f[L + 1];
+    boys<L, R>(R(T[i]), f, lds);
+    for (int m = 0; m <= L; ++m) F[(size_t)i * (L + 1) + m] = (double)f[m];
+  }
+}
+
+template <typename R>
+int launch_boys_eval(int L, const double* tab, const double* T, double* F, int nT, hipStream_t st) {
+  const unsigned grid = (unsigned)std::max(1, std::min(1024, (nT + 255) / 256));
+  switch (L) {
+#define SQJK_BOYS_CASE(l)                                                                                    \
+  case l:                                                                                                    \
+    hipLaunchKernelGGL((k_boys_eval<l, R>), dim3(grid), dim3(256), sizeof(double) * boys_rows<l>() * kBoysLdsW, \
+                       st, tab, T, F, nT);                                                                   \
+    return 0;
+    SQJK_BOYS_CASE(0) SQJK_BOYS_CASE(1) SQJK_BOYS_CASE(2) SQJK_BOYS_CASE(3) SQJK_BOYS_CASE(4)
+    SQJK_BOYS_CASE(5) SQJK_BOYS_CASE(6) SQJK_BOYS_CASE(7) SQJK_BOYS_CASE(8)
+#undef SQJK_BOYS_CASE
+    default: return -1;
+  }
+}
+
 }  // namespace
 
 struct sqjk_ctx {
@@ -262,6 +331,8 @@ struct sqjk_ctx {
   // per class pair (X,Y) of the last build: contracted and primitive quartets
   long long cls_q[kNPairClass][kNPairClass] = {};
   long long cls_p[kNPairClass][kNPairClass] = {};
+  double last_fixed = 0.0;  // density-weighted cut-off of the last build
+  bool built = false;       // a build has run (PM holds its density maxima)
 };
 
 namespace {
@@ -1036,6 +1107,40 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
   c->last.quartets = c->plan_q;
   c->last.quartets_fp32 = c->plan_q32;
   c->last.launches = launches;
+  c->last_fixed = fixedCutoff;
+  c->built = true;
+  return SQJK_OK;
+}
+
+int sqjk_boys(int L, int nT, const double* T, int fp32, int device, double* F) {
+  if (L < 0 || L > 8 || nT < 0 || (nT > 0 && (!T || !F))) return fail(SQJK_EARG, "sqjk_boys: invalid arguments");
+  if (nT == 0) return SQJK_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(SQJK_ENODEV, "no HIP device is visible; this library has no CPU path");
+  if (device < 0 || device >= ndev) return fail(SQJK_EARG, "sqjk_boys: device ordinal out of range");
+  HIPCHK(hipSetDevice(device));
+  for (int i = 0; i < nT; ++i)
+    if (!(T[i] >= 0.0)) return fail(SQJK_EARG, "sqjk_boys: T must be >= 0");
+  const std::vector<double>& boys = boys_table();
+  double *d_tab = nullptr, *d_T = nullptr, *d_F = nullptr;
+  int rc;
+  if ((rc = dalloc(&d_tab, boys.size())) || (rc = dalloc(&d_T, (size_t)nT)) ||
+      (rc = dalloc(&d_F, (size_t)nT * (L + 1)))) {
+    block_put(d_tab); block_put(d_T); block_put(d_F);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_tab, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_T, T, (size_t)nT * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const int k = fp32 ? launch_boys_eval<float>(L, d_tab, d_T, d_F, nT, 0)
+                       : launch_boys_eval<double>(L, d_tab, d_T, d_F, nT, 0);
+    e = k ? hipErrorInvalidValue : hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(F, d_F, (size_t)nT * (L + 1) * sizeof(double), hipMemcpyDeviceToHost);
+  block_put(d_tab); block_put(d_T); block_put(d_F);
+  if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_boys: ") + hipGetErrorString(e));
   return SQJK_OK;
 }
 
@@ -1068,6 +1173,53 @@ int sqjk_class_stats(const sqjk_ctx* c, int* lq, long long* quartets, long long*
       }
       if (quartets) quartets[k] = c->cls_q[X][Y];
       if (prim_quartets) prim_quartets[k] = c->cls_p[X][Y];
+    }
+  return k;
+}
+
+int sqjk_class_work(sqjk_ctx* c, long long* quartets, long long* prim_quartets, long long* quartets_fp32,
+                    long long* prim_quartets_fp32) {
+  if (!c) return fail(SQJK_EARG, "sqjk_class_work: null context");
+  if (!c->built) return fail(SQJK_EARG, "sqjk_class_work: no build has run on this context");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nl = c->launches.size();
+  unsigned long long* d_cnt = nullptr;
+  int rc = dalloc(&d_cnt, 2 * std::max<size_t>(nl, 1));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->st, reinterpret_cast<double*>(d_cnt), 2 * nl, 0.0);
+  for (size_t li = 0; li < nl; ++li) {
+    const sqjk_ctx::Launch& L = c->launches[li];
+    ClassArgs a = base_args(c);
+    a.bra = c->d_pairs[L.X];
+    a.ket = c->d_pairs[L.Y];
+    a.seg = c->d_seg;
+    a.tile = c->d_tile + L.tile0;
+    a.rbra = c->d_rbra; a.rlo = c->d_rlo; a.rhi = c->d_rhi;
+    a.total = L.ntile;
+    a.fixed_cut = c->last_fixed;
+    const int grid = (int)std::min<long long>((L.ntile + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_count_work, dim3(std::max(grid, 1)), dim3(256), 0, c->st, a, d_cnt + 2 * li);
+  }
+  std::vector<unsigned long long> h(2 * nl);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && nl) e = hipMemcpyAsync(h.data(), d_cnt, 2 * nl * sizeof(unsigned long long),
+                                                hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  block_put(d_cnt);
+  if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_class_work: ") + hipGetErrorString(e));
+  long long q[kNPairClass][kNPairClass][2] = {}, p[kNPairClass][kNPairClass][2] = {};
+  for (size_t li = 0; li < nl; ++li) {
+    const sqjk_ctx::Launch& L = c->launches[li];
+    q[L.X][L.Y][L.fp32] += (long long)h[2 * li];
+    p[L.X][L.Y][L.fp32] += (long long)h[2 * li + 1];
+  }
+  int k = 0;
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y <= X; ++Y, ++k) {
+      if (quartets) quartets[k] = q[X][Y][0];
+      if (prim_quartets) prim_quartets[k] = p[X][Y][0];
+      if (quartets_fp32) quartets_fp32[k] = q[X][Y][1];
+      if (prim_quartets_fp32) prim_quartets_fp32[k] = p[X][Y][1];
     }
   return k;
 }

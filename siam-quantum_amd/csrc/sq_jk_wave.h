@@ -136,14 +136,16 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
   // tile one after the other; Schwarz: grid-stride over the pairs.
   const long long items = MODE == kSchwarz ? A.total : A.total * 64;
   for (long long t = blockIdx.x; t < items; t += gridDim.x) {
-    int r0 = 0, r1 = 1, j = 0, nppb = 0, nppk = 0, ket0 = 0;
+    // ket primitive pairs [v0, v1): the whole range in Schwarz mode, the
+    // tile's share of a segment split along its ket primitive pairs otherwise
+    int r0 = 0, r1 = 1, j = 0, nppb = 0, v0 = 0, v1 = 0, ket0 = 0;
     if constexpr (MODE == kSchwarz) {
-      nppb = nppk = uload(A.bra + t).npp;
+      nppb = v1 = uload(A.bra + t).npp;
     } else {
       const Tile tl = uload(A.tile + (t >> 6));
       const Segment sg = uload(A.seg + tl.seg);
       r0 = tl.r0; r1 = tl.r1; j = tl.k0 + (int)(t & 63);
-      nppb = sg.nppb; nppk = sg.nppk; ket0 = sg.ket0;
+      nppb = sg.nppb; v0 = tl.v0; v1 = tl.v1; ket0 = sg.ket0;
     }
     for (int r = r0; r < r1; ++r) {
     int ib, ik;
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
 
     for (int u = 0; u < nppb; ++u) {
       const PrimPair P = uload(A.pp + (bp.pp_off + u * bp.pp_stride));
-      for (int v = 0; v < nppk; ++v) {
+      for (int v = v0; v < v1; ++v) {
         const PrimPair Q = uload(A.pp + (kp.pp_off + v * kp.pp_stride));
         const double s = P.p + Q.p;
         const double rs = rsqrt_nr(s);
@@ -214,7 +216,7 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
         for (int e = lane; e < W_::NW; e += kWT) {
           const int cd = e / W_::NHAB, h = e - cd * W_::NHAB;
           const Cart c = cart(LC, cd / ND), d = cart(LD, cd % ND);
-          const int hd = hdec[h], t0 = hd & 255, u0 = (hd >> 8) & 255, v0 = hd >> 16;
+          const int hd = hdec[h], t0 = hd & 255, u0 = (hd >> 8) & 255, w0 = hd >> 16;
           const R* ex = Ec + 0 * W_::EC + (c.x * (LD + 1) + d.x) * (LC + LD + 1);
           const R* ey = Ec + 1 * W_::EC + (c.y * (LD + 1) + d.y) * (LC + LD + 1);
           const R* ez = Ec + 2 * W_::EC + (c.z * (LD + 1) + d.z) * (LC + LD + 1);
@@ -224,7 +226,7 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
               R wz = R(0);
               for (int V_ = 0; V_ <= c.z + d.z; ++V_) {
                 const R sg = ((T_ + U_ + V_) & 1) ? R(-1) : R(1);
-                wz += sg * ez[V_] * Rh[hidx(t0 + T_, u0 + U_, v0 + V_)];
+                wz += sg * ez[V_] * Rh[hidx(t0 + T_, u0 + U_, w0 + V_)];
               }
               w += ex[T_] * ey[U_] * wz;
             }

@@ -42,6 +42,8 @@ def lib():
     L.sqjk_class_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.sqjk_class_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)]
+    PLL = C.POINTER(C.c_longlong)
+    L.sqjk_class_work.argtypes = [C.c_void_p, PLL, PLL, PLL, PLL]
     L.sqjk_errmsg.restype = C.c_char_p
     L.GTO_JK_Matrix_Quartet_Parallel.argtypes = [C.c_int, C.c_int, PD, PD, PG, PD, C.c_double,
                                                  PD, PD, PD, C.POINTER(Option)]
@@ -101,6 +103,17 @@ class JKEngine:
         p = (C.c_longlong * 21)()
         k = lib().sqjk_class_stats(self._h, lq, q, p)
         return [((lq[4 * i], lq[4 * i + 1], lq[4 * i + 2], lq[4 * i + 3]), q[i], p[i]) for i in range(k)]
+
+    def class_work(self):
+        """Surviving work of the last build per class: [(la,lb,lc,ld), quartets,
+        primitive quartets, quartets in FP32, primitive quartets in FP32],
+        after the density screen and the primitive cut-off."""
+        q, p, q32, p32 = [(C.c_longlong * 21)() for _ in range(4)]
+        k = lib().sqjk_class_work(self._h, q, p, q32, p32)
+        if k < 0:
+            _check(k, "sqjk_class_work")
+        lq = [c[0] for c in self.class_stats()]
+        return [(lq[i], q[i], p[i], q32[i], p32[i]) for i in range(k)]
 
     def set_profile(self, on=True):
         _check(lib().sqjk_set_profile(self._h, int(on)), "sqjk_set_profile")

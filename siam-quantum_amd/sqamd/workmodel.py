@@ -85,3 +85,19 @@ def build_flops(class_stats, uhf=False):
 
 def integrals(class_stats):
     return sum(nq * ncart(a) * ncart(b) * ncart(c) * ncart(d) for (a, b, c, d), nq, _ in class_stats)
+
+
+def nint(la, lb, lc, ld):
+    """Contracted Cartesian integrals per quartet of the class."""
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+
+
+def class_block(la, lb, lc, ld):
+    """Workgroup size of the per-thread class kernel (class_block in
+    csrc/sq_jk_dev.h): 512 lanes when a lane holds at most 9 integrals of its
+    ket-component slice, else 256."""
+    nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
+    per = max(36 // nab, 1)
+    nslice = (ncd + per - 1) // per
+    sz = (ncd + nslice - 1) // nslice
+    return 512 if nab * sz <= 9 else 256

@@ -4,11 +4,13 @@ src/matrix.c:3773-3986 and src/int.c), on seeded densities.
 Tolerances: the plain FP64 path must reproduce the reference Fock pieces to
 1e-9 max-abs; the cases here hold 1e-10.
 """
+import os
+
 import numpy as np
 import pytest
 
-from molecules import cluster, g2, random_density, water
-from sqamd import JKEngine, default_option, dptr, lib
+from molecules import G2, cluster, g2, random_density, water
+from sqamd import JKEngine, default_option, dptr, inputs, lib
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -86,3 +88,18 @@ def test_dropin_accumulates_and_resets(oracle):
     assert np.abs(J - (Jo + 1.0)).max() < TOL
     assert np.abs(KA - (KAo + 1.0)).max() < TOL
     assert np.array_equal(KA, KB)
+
+
+D3 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "basis", "d3_test.txt")
+
+
+@pytest.mark.parametrize("restricted", [True, False])
+def test_contracted_d_split_tiles(oracle, restricted):
+    """Three-primitive d shells (a synthetic GAMESS basis, tests/golden/basis):
+    a (dd|dd) quartet then has 81 primitive quartets, above the split limit of
+    its class, so its tiles are cut along the ket primitive pairs; both the
+    cooperative (dd|dd) kernel and the per-thread d classes must sum each
+    primitive range once."""
+    mol = inputs.read_xyz(os.path.join(G2, "ch2o_q0m1.xyz"))
+    bas = inputs.make_basis(mol, D3)
+    _compare(oracle, mol, bas, restricted, seed=4)
