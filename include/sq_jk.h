@@ -52,6 +52,15 @@ typedef struct {
 int sqjk_create(int nBasis, const struct GTOBasis_t *gto, const double *schwarz_basis,
                 int device, sqjk_ctx **out);
 
+/* Moves the context to a new geometry of the same basis (same functions,
+ * exponents and coefficients; only the centres differ), as in the
+ * geometry steps of an optimisation or molecular dynamics run
+ * (src/qmd.c:564-667, src/optimize.c:283-340): primitive pair data and the
+ * Schwarz bounds are recomputed on the device, while the shells, pair
+ * lists, device buffers and stream of the context are kept.  Returns
+ * SQJK_EARG if the basis differs in anything but its centres. */
+int sqjk_update_geometry(sqjk_ctx *ctx, const struct GTOBasis_t *gto);
+
 /* One J/K build on density PA (alpha) and PB (beta):
  *   JT += J[PA+PB],  KA -= K[PA],  KB -= K[PB]   (accumulating, as the
  * reference does), then KB := KA when restricted, then upper triangle :=
@@ -62,6 +71,19 @@ int sqjk_create(int nBasis, const struct GTOBasis_t *gto, const double *schwarz_
  * primitive quartets with a bare prefactor below 1e-15 are dropped. */
 int sqjk_build(sqjk_ctx *ctx, const double *PA, const double *PB, double fixedCutoff,
                double scfCutoff, int restricted, double *JT, double *KA, double *KB);
+
+/* Device-resident variant of sqjk_build for callers that keep their matrices
+ * in HIP device memory (an SCF that stays on the device, batched and served
+ * solves): dPA, dPB are n x n device densities; dJ, dKA, dKB (device) are
+ * OVERWRITTEN with the full symmetric J[PA+PB], -K[PA], -K[PB] (dKB := dKA
+ * when restricted), no accumulation.  Same screening as sqjk_build.  The
+ * work is queued on the context's stream and the call returns without
+ * waiting; sqjk_sync waits for it.  The inputs must stay valid and the
+ * outputs untouched until then.  No counterpart in the reference. */
+int sqjk_build_device(sqjk_ctx *ctx, const double *dPA, const double *dPB, double fixedCutoff,
+                      double scfCutoff, int restricted, double *dJ, double *dKA, double *dKB);
+/* Waits for the context's queued work (and reads its timing). */
+int sqjk_sync(sqjk_ctx *ctx);
 
 /* Mixed precision: shell quartets whose Schwarz product Sch_PQ*Sch_IJ is
  * below `threshold` are evaluated in FP32 (Boys, Hermite and contraction)
@@ -146,6 +168,7 @@ void GTO_JK_Matrix_Quartet_Parallel(int childID, int nBasis, const double *PA, c
  * uhf()); 0 (default) is the plain FP64 path.  No counterpart in the
  * reference. */
 void sqjk_set_dropin_mixed(double threshold);
+double sqjk_get_dropin_mixed(void);
 
 /* src/uhf.h:26-33: zero JT, KA, KB, then one J/K build. */
 void uhf_getGMatrix(int nBasis, const struct GTOBasis_t *gto, const double *Schwarz,

@@ -27,9 +27,58 @@ void rhf_getDMatrix(int nBasis, int nOcc, double *C, double *P);
 double rhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nE,
            double *C, double *e, struct option_t *opt);
 
-/* ---- convergence (src/conv.h:34-37, src/conv.c:606-844) ----
- * conv_diis4(0, 0.0, NULL, NULL) resets the history. */
+/* ---- convergence (src/conv.h:22-37, src/conv.c:40-844) ----
+ * Density damping and 2-, 3-, 4-point density DIIS; each keeps its own
+ * history, and a call (0, 0.0, NULL, NULL) resets it. */
+double conv_damping(int nBasis, double alpha, double *PA, double *PB);
+double conv_diis2(int nBasis, double alpha, double *PA, double *PB);
+double conv_diis3(int nBasis, double alpha, double *PA, double *PB);
 double conv_diis4(int nBasis, double alpha, double *PA, double *PB);
+
+/* ---- SCF with its state on the device ----
+ * The iteration of uhf() (src/uhf.c:1464-1783) with H, S, the densities,
+ * the accumulated J/K, the Fock matrices, the DIIS history (any
+ * opt->convMethod) and the orbitals held on the HIP device; per iteration
+ * only the energy, the rms density change and the DIIS scalars reach the
+ * host.  uhf() runs on it.  A context keeps its state across solves, so a
+ * run of geometry steps reuses the J/K context (sqjk_update_geometry) and
+ * can start from the previous orbitals (the CACB guess of src/qmd.c:585).
+ * No counterpart in the reference, whose state is host memory. */
+struct sqjk_ctx;
+typedef struct sqscf_ctx sqscf_ctx;
+typedef struct {
+	int restricted, nEA, nEB;
+	double SCFConv;
+	int SCFMax;
+	double SCFDrag;
+	int threeStep;          /* 3-step cut-offs 1e-3/1e-6/1e-9 (SCFACCURACY_3STEP) */
+	int convMethod;         /* CONVMETHOD_DAMPING / DIIS2 / DIIS3 / DIIS4 */
+	double Econst;          /* nuclear repulsion and field terms of the energy */
+	double mixedThreshold;  /* FP32 below this Schwarz product (0: FP64) */
+	int eigenEveryIter;     /* orbitals every iteration (else only at the end) */
+	/* called after every iteration (may read sqscf_orbitals when
+	 * eigenEveryIter is set); NULL for none */
+	void (*onIter)(void *user, sqscf_ctx *ctx, int iter, double E, double dE, double rmsDP);
+	void *user;
+} sqscf_params;
+typedef struct {
+	double energy, dE, rmsDP, realCutoff, seconds;
+	int iterations, converged;
+} sqscf_result;
+int sqscf_create(int device, sqscf_ctx **out);
+/* H = T + V (+ field) and S of the basis, host arrays.  The same basis at a
+ * new geometry keeps the J/K context. */
+int sqscf_set_system(sqscf_ctx *ctx, int nBasis, const struct GTOBasis_t *gto, const double *H, const double *S);
+int sqscf_set_density(sqscf_ctx *ctx, const double *PA, const double *PB);
+/* Initial density from the orbitals of the last solve, S-normalised in the
+ * current S (SCFGUESS_CACB, src/uhf.c:1358-1364). */
+int sqscf_guess_from_orbitals(sqscf_ctx *ctx, int nEA, int nEB);
+int sqscf_run(sqscf_ctx *ctx, const sqscf_params *prm, sqscf_result *res);
+int sqscf_orbitals(sqscf_ctx *ctx, double *CA, double *CB, double *eA, double *eB);
+int sqscf_density(sqscf_ctx *ctx, double *PA, double *PB);
+int sqscf_jk_context(sqscf_ctx *ctx, struct sqjk_ctx **jk);
+void sqscf_destroy(sqscf_ctx *ctx);
+const char *sqscf_errmsg(void);
 
 /* ---- dense linear algebra (src/lin.h:20-33) ---- */
 int gen_sym_eigen(int nDim, const double *A, const double *S, double *e, double *C);

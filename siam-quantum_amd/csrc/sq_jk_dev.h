@@ -21,9 +21,10 @@ namespace sqjk {
 // one pure-angular-momentum shell (a Pople L shell becomes an s and a p)
 struct SubShell {
   double x, y, z;
-  int l, nprim, prim_off, bf_off;  // first basis function
+  int l, nprim, prim_off, bf_off;  // first basis function (of its molecule)
   int parent;                       // reference shell (screening unit)
-  int pad;
+  int gbf;                          // first basis function in the component
+                                    // scales (bscale), across a batch
 };
 
 // Unique shell pair, a = higher angular momentum.  Primitive pair v of the
@@ -51,7 +52,7 @@ struct __attribute__((aligned(16))) PrimPair {
 struct Segment {
   int ket0;          // class-list index of the first ket pair of the ket group
   int nppb, nppk;    // primitive pairs of every bra / ket pair of the segment
-  int pad;
+  int inst;          // molecule of the segment in a batched launch (0 otherwise)
 };
 
 // Row r: bra pair rbra[r] (class-list index) with the ket pairs at positions
@@ -65,6 +66,25 @@ struct Tile {
 };
 
 enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
+
+// Per-molecule data of a batched launch (one entry per molecule): its
+// matrices, screening maxima, half-accumulators and, per ket pair class,
+// its exchange staging and the range of its ket pairs in the class list.
+struct InstDesc {
+  const double* DJ;
+  const double* DA;
+  const double* DB;
+  const double* PM;
+  double* AJ;
+  double* AKA;
+  double* AKB;
+  double* SA[6];
+  double* SB[6];
+  int np[6];          // ket pairs of the molecule in each pair class
+  int p0[6];          // first of them in the class list
+  int n, nref;
+  double fixed_cut;
+};
 
 // Load of read-only work-list data through the constant address space: the
 // compiler may then assume the memory is not written during the kernel, so a
@@ -123,7 +143,46 @@ struct ClassArgs {
   long long total;                    // tiles (RHF/UHF), pairs (Schwarz)
   double fixed_cut;                   // density-weighted screening cut
   int same, n, nref;
+  // batched launch: per-molecule data (nullptr for one molecule) and the
+  // ket pair class of the launch
+  const InstDesc* inst;
+  int ycls;
 };
+
+// The per-molecule part of the arguments, as one tile sees it: from the
+// arguments themselves for a single molecule, from the tile's molecule
+// entry in a batched launch (wavefront-uniform scalar loads).  S holds the
+// exchange staging of the molecule's ket pairs [k0, k0 + npk) of the class.
+struct View {
+  const double* DJ;
+  const double* DA;
+  const double* DB;
+  const double* PM;
+  double* AJ;
+  double* AKA;
+  double* AKB;
+  double* SA;
+  double* SB;
+  int n, nref, npk, k0;
+  double fixed_cut;
+};
+
+__host__ __device__ __forceinline__ View make_view(const ClassArgs& A, int inst) {
+  View v;
+  if (A.inst == nullptr) {
+    v.DJ = A.DJ; v.DA = A.DA; v.DB = A.DB; v.PM = A.PM;
+    v.AJ = A.AJ; v.AKA = A.AKA; v.AKB = A.AKB; v.SA = A.SA; v.SB = A.SB;
+    v.n = A.n; v.nref = A.nref; v.npk = A.npk; v.k0 = 0; v.fixed_cut = A.fixed_cut;
+  } else {
+    const InstDesc* d = A.inst + inst;
+    v.DJ = uload(&d->DJ); v.DA = uload(&d->DA); v.DB = uload(&d->DB); v.PM = uload(&d->PM);
+    v.AJ = uload(&d->AJ); v.AKA = uload(&d->AKA); v.AKB = uload(&d->AKB);
+    v.SA = uload(&d->SA[A.ycls]); v.SB = uload(&d->SB[A.ycls]);
+    v.n = uload(&d->n); v.nref = uload(&d->nref); v.npk = uload(&d->np[A.ycls]); v.k0 = uload(&d->p0[A.ycls]);
+    v.fixed_cut = uload(&d->fixed_cut);
+  }
+  return v;
+}
 
 using namespace sqmd;
 

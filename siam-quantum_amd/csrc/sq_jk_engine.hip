@@ -37,6 +37,7 @@
 #include "../../include/sq_jk.h"
 #include "sq_jk_classes.h"
 #include "sq_one_e.h"
+#include "sq_internal.h"
 
 using namespace sqjk;
 
@@ -332,6 +333,7 @@ struct sqjk_ctx {
   long long cls_q[kNPairClass][kNPairClass] = {};
   long long cls_p[kNPairClass][kNPairClass] = {};
   double last_fixed = 0.0;  // density-weighted cut-off of the last build
+  bool timing_pending = false;  // the last build's events are not yet read
   bool built = false;       // a build has run (PM holds its density maxima)
 };
 
@@ -892,6 +894,48 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   return SQJK_OK;
 }
 
+int sqjk_update_geometry(sqjk_ctx* c, const GTOBasis_t* gto) {
+  if (!c || !gto) return fail(SQJK_EARG, "sqjk_update_geometry: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  // the new basis must have the same shells, primitives and coefficients;
+  // only the centres may move
+  sqjk_ctx t;
+  std::vector<double> pexp, pcoef;
+  int rc = plan_shells(&t, c->n, gto, pexp, pcoef);
+  if (rc) return rc;
+  bool same = t.nref == c->nref && t.nsub == c->nsub;
+  for (int r = 0; same && r < c->nref; ++r) same = t.ref[r].first == c->ref[r].first && t.ref[r].count == c->ref[r].count;
+  for (int i = 0; same && i < c->nsub; ++i) {
+    const SubShell &a = t.sub[i], &b = c->sub[i];
+    same = a.l == b.l && a.nprim == b.nprim && a.bf_off == b.bf_off && a.parent == b.parent;
+  }
+  for (int f = 0; same && f < c->n; ++f) same = t.bscale[f] == c->bscale[f];
+  if (!same) return fail(SQJK_EARG, "sqjk_update_geometry: the basis differs in more than its centres");
+  // new centres: primitive pairs (pair order), Schwarz on the device, then
+  // groups and the primitive-major layout as at creation; every buffer and
+  // the stream are kept, the work plan is rebuilt at the next build
+  c->sub = t.sub;
+  plan_pairs(c, pexp, pcoef);
+  HIPCHK(hipMemcpyAsync(c->d_sub, c->sub.data(), c->sub.size() * sizeof(SubShell), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st));
+  for (int X = 0; X < kNPairClass; ++X)
+    if (!c->pairs[X].empty())
+      HIPCHK(hipMemcpyAsync(c->d_pairs[X], c->pairs[X].data(), c->pairs[X].size() * sizeof(PairRec),
+                            hipMemcpyHostToDevice, c->st));
+  if ((rc = compute_schwarz(c))) return rc;
+  finish_pairs(c);
+  for (int X = 0; X < kNPairClass; ++X)
+    if (!c->pairs[X].empty())
+      HIPCHK(hipMemcpyAsync(c->d_pairs[X], c->pairs[X].data(), c->pairs[X].size() * sizeof(PairRec),
+                            hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->plan_ok = false;
+  c->built = false;
+  return SQJK_OK;
+}
+
 int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, int device, double* T, double* V,
                       double* S) {
   if (nBasis <= 0 || !gto || !mol || mol->nAtom < 0) return fail(SQJK_EARG, "sqjk_one_electron: invalid arguments");
@@ -955,33 +999,17 @@ int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, 
   return SQJK_OK;
 }
 
-int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCutoff, double scfCutoff,
-               int restricted, double* JT, double* KA, double* KB) {
-  if (!c || !PA || !PB || !JT || !KA || !KB) return fail(SQJK_EARG, "sqjk_build: null argument");
-  const auto t0 = std::chrono::steady_clock::now();
-  HIPCHK(hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+
+// The device part of a build.  The densities are in c->d_DA, c->d_DB; the
+// results J and -K (full, symmetric) are written to oJ, oKA and, for UHF,
+// oKB.  Everything is queued on the context's stream; nothing waits.
+int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, double* oJ, double* oKA,
+                    double* oKB) {
   const int n = c->n, nr = c->nref;
   const size_t nn = (size_t)n * n;
-  const int uhf = restricted ? 0 : 1;
-  // densities: J uses PA+PB, K uses PA (and PB).  Both go up through the
-  // pinned staging buffers (the previous build has synchronised, so they are
-  // free); D_J and the screening maxima are formed on the device.
-  // The copy of P_B into its staging buffer overlaps the upload of P_A; a
-  // caller passing the same matrix twice gets one upload and a device copy.
-  auto stage = [&](double* dst, const double* src) {
-    host_parallel(8, [&](int t, int nt) {
-      const size_t e0 = nn * t / nt, e1 = nn * (t + 1) / nt;
-      std::memcpy(dst + e0, src + e0, (e1 - e0) * sizeof(double));
-    });
-  };
-  stage(c->hp_A, PA);
-  HIPCHK(hipMemcpyAsync(c->d_DA, c->hp_A, nn * 8, hipMemcpyHostToDevice, c->st));
-  if (PB == PA) {
-    HIPCHK(hipMemcpyAsync(c->d_DB, c->d_DA, nn * 8, hipMemcpyDeviceToDevice, c->st));
-  } else {
-    stage(c->hp_B, PB);
-    HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
-  }
   hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
                      c->d_DJ, nn);
   hipLaunchKernelGGL(k_pair_max, dim3(grid_for((long long)nr * nr, 256)), dim3(256), 0, c->st, c->d_DJ, c->d_DA,
@@ -1063,9 +1091,70 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
                        c->d_AKB, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev1, c->st));
-  hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, c->d_J,
-                     c->d_KA, c->d_KB, n, uhf);
+  hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, oJ, oKA, oKB,
+                     n, uhf);
   HIPCHK(hipGetLastError());
+  c->last.quartets = c->plan_q;
+  c->last.quartets_fp32 = c->plan_q32;
+  c->last.launches = launches;
+  c->last_fixed = fixedCutoff;
+  c->built = true;
+  c->timing_pending = true;
+  return SQJK_OK;
+}
+
+// Kernel time of the last build from its events (the build must have
+// completed), and the per-class event times in profile mode.
+int collect_times(sqjk_ctx* c) {
+  if (!c->timing_pending) return SQJK_OK;
+  float ms = 0.f;
+  HIPCHK(hipEventSynchronize(c->ev1));
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  std::memset(c->cls_ms, 0, sizeof(c->cls_ms));
+  if (c->profile)
+    for (size_t li = 0; li < c->launches.size(); ++li) {
+      float lm = 0.f;
+      HIPCHK(hipEventElapsedTime(&lm, c->prof_ev[2 * li], c->prof_ev[2 * li + 1]));
+      c->cls_ms[c->launches[li].X][c->launches[li].Y] += lm;
+    }
+  c->last.kernel_ms = ms;
+  c->timing_pending = false;
+  return SQJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCutoff, double scfCutoff,
+               int restricted, double* JT, double* KA, double* KB) {
+  if (!c || !PA || !PB || !JT || !KA || !KB) return fail(SQJK_EARG, "sqjk_build: null argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(c->device));
+  const int n = c->n;
+  const size_t nn = (size_t)n * n;
+  const int uhf = restricted ? 0 : 1;
+  // densities: J uses PA+PB, K uses PA (and PB).  Both go up through the
+  // pinned staging buffers (the previous build has synchronised, so they are
+  // free); D_J and the screening maxima are formed on the device.
+  // The copy of P_B into its staging buffer overlaps the upload of P_A; a
+  // caller passing the same matrix twice gets one upload and a device copy.
+  auto stage = [&](double* dst, const double* src) {
+    host_parallel(8, [&](int t, int nt) {
+      const size_t e0 = nn * t / nt, e1 = nn * (t + 1) / nt;
+      std::memcpy(dst + e0, src + e0, (e1 - e0) * sizeof(double));
+    });
+  };
+  stage(c->hp_A, PA);
+  HIPCHK(hipMemcpyAsync(c->d_DA, c->hp_A, nn * 8, hipMemcpyHostToDevice, c->st));
+  if (PB == PA) {
+    HIPCHK(hipMemcpyAsync(c->d_DB, c->d_DA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  } else {
+    stage(c->hp_B, PB);
+    HIPCHK(hipMemcpyAsync(c->d_DB, c->hp_B, nn * 8, hipMemcpyHostToDevice, c->st));
+  }
+  int rc = build_on_device(c, fixedCutoff, scfCutoff, uhf, c->d_J, c->d_KA, c->d_KB);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->hp_J, c->d_J, nn * 8, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->hp_KA, c->d_KA, nn * 8, hipMemcpyDeviceToHost, c->st));
   if (uhf) HIPCHK(hipMemcpyAsync(c->hp_KB, c->d_KB, nn * 8, hipMemcpyDeviceToHost, c->st));
@@ -1093,23 +1182,31 @@ int sqjk_build(sqjk_ctx* c, const double* PA, const double* PB, double fixedCuto
         }
     }
   });
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  std::memset(c->cls_ms, 0, sizeof(c->cls_ms));
-  if (c->profile)
-    for (size_t li = 0; li < c->launches.size(); ++li) {
-      float lm = 0.f;
-      HIPCHK(hipEventElapsedTime(&lm, c->prof_ev[2 * li], c->prof_ev[2 * li + 1]));
-      c->cls_ms[c->launches[li].X][c->launches[li].Y] += lm;
-    }
-  c->last.kernel_ms = ms;
+  if ((rc = collect_times(c))) return rc;
   c->last.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  c->last.quartets = c->plan_q;
-  c->last.quartets_fp32 = c->plan_q32;
-  c->last.launches = launches;
-  c->last_fixed = fixedCutoff;
-  c->built = true;
   return SQJK_OK;
+}
+
+int sqjk_build_device(sqjk_ctx* c, const double* dPA, const double* dPB, double fixedCutoff, double scfCutoff,
+                      int restricted, double* dJ, double* dKA, double* dKB) {
+  if (!c || !dPA || !dPB || !dJ || !dKA || !dKB) return fail(SQJK_EARG, "sqjk_build_device: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nn = (size_t)c->n * c->n;
+  const int uhf = restricted ? 0 : 1;
+  HIPCHK(hipMemcpyAsync(c->d_DA, dPA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_DB, dPB, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  int rc = build_on_device(c, fixedCutoff, scfCutoff, uhf, dJ, dKA, dKB);
+  if (rc) return rc;
+  if (!uhf && dKB != dKA) HIPCHK(hipMemcpyAsync(dKB, dKA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  c->last.total_ms = 0.0;
+  return SQJK_OK;
+}
+
+int sqjk_sync(sqjk_ctx* c) {
+  if (!c) return fail(SQJK_EARG, "sqjk_sync: null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return collect_times(c);
 }
 
 int sqjk_boys(int L, int nT, const double* T, int fp32, int device, double* F) {
@@ -1158,6 +1255,8 @@ int sqjk_schwarz(sqjk_ctx* c, double* out) {
 
 int sqjk_last_stats(const sqjk_ctx* c, sqjk_stats* st) {
   if (!c || !st) return fail(SQJK_EARG, "sqjk_last_stats: null argument");
+  int rc = collect_times(const_cast<sqjk_ctx*>(c));
+  if (rc) return rc;
   *st = c->last;
   return SQJK_OK;
 }
@@ -1232,6 +1331,7 @@ int sqjk_set_profile(sqjk_ctx* c, int on) {
 
 int sqjk_class_times(const sqjk_ctx* c, double* kernel_ms) {
   if (!c || !kernel_ms) return fail(SQJK_EARG, "sqjk_class_times: null argument");
+  if (int rc = collect_times(const_cast<sqjk_ctx*>(c))) return rc;
   int k = 0;
   for (int X = 0; X < kNPairClass; ++X)
     for (int Y = 0; Y <= X; ++Y, ++k) kernel_ms[k] = c->cls_ms[X][Y];
@@ -1274,6 +1374,15 @@ void sqjk_destroy(sqjk_ctx* c) {
 // ------------------------------------------------------------ drop-ins
 static sqjk_ctx* g_ctx = nullptr;  // the reference keeps this state in statics too
 static double g_dropin_mixed = 0.0;
+
+}  // extern "C"
+
+hipStream_t sqjk_stream_internal(sqjk_ctx* c) { return c ? c->st : nullptr; }
+double sqjk_dropin_mixed_internal() { return g_dropin_mixed; }
+
+extern "C" {
+
+double sqjk_get_dropin_mixed(void) { return g_dropin_mixed; }
 
 void sqjk_set_dropin_mixed(double threshold) {
   g_dropin_mixed = threshold > 0.0 ? threshold : 0.0;

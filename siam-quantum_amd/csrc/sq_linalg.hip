@@ -1,7 +1,8 @@
 // sq_linalg.hip - dense FP64 linear algebra of the SCF on the device.
 //
 // Generalised symmetric eigenproblem F C = S C e on the device, behind the
-// reference's src/lin.h surface.  S is fixed during an SCF, so its Cholesky
+// reference's src/lin.h surface, and the same operations on device-resident
+// matrices for the SCF that stays on the device (sqla:: in sq_linalg_dev.h).  S is fixed during an SCF, so its Cholesky
 // factor S = U^T U and X = U^-1 are computed once per metric (rocSOLVER
 // dpotrf/dtrtri) and every solve is X^T F X (two DGEMMs), rocSOLVER dsyevd
 // and C = X C' (one DGEMM):
@@ -23,40 +24,9 @@
 #include <vector>
 
 #include "../../include/sq_scf.h"
+#include "sq_linalg_dev.h"
 
 namespace {
-
-struct EigWork {
-  int n = 0;
-  rocblas_handle h = nullptr;
-  double *A = nullptr, *B = nullptr, *W = nullptr, *E = nullptr, *P = nullptr, *X = nullptr;
-  double* red = nullptr;  // Gershgorin bounds and traces of the purification
-  rocblas_int* info = nullptr;
-  int metric_n = 0;             // X = U^-1 is valid for this n and the host copy below
-  std::vector<double> metric;   // S of the cached factor
-};
-std::mutex g_mu;
-EigWork g_w;
-
-int ensure(int n) {
-  if (!g_w.h && rocblas_create_handle(&g_w.h) != rocblas_status_success) return -1;
-  if (g_w.n >= n) return 0;
-  if (g_w.A) {
-    (void)hipFree(g_w.A); (void)hipFree(g_w.B); (void)hipFree(g_w.W); (void)hipFree(g_w.E);
-    (void)hipFree(g_w.P); (void)hipFree(g_w.X); (void)hipFree(g_w.info); (void)hipFree(g_w.red);
-  }
-  g_w.metric_n = 0;
-  const size_t nn = (size_t)n * n;
-  if (hipMalloc(&g_w.A, nn * 8) != hipSuccess || hipMalloc(&g_w.B, nn * 8) != hipSuccess ||
-      hipMalloc(&g_w.P, nn * 8) != hipSuccess || hipMalloc(&g_w.X, nn * 8) != hipSuccess ||
-      hipMalloc(&g_w.W, (size_t)n * 8) != hipSuccess || hipMalloc(&g_w.E, (size_t)n * 8) != hipSuccess ||
-      hipMalloc(&g_w.info, sizeof(rocblas_int)) != hipSuccess || hipMalloc(&g_w.red, 4 * sizeof(double)) != hipSuccess) {
-    g_w.n = 0;
-    return -1;
-  }
-  g_w.n = n;
-  return 0;
-}
 
 // strictly lower triangle (column-major) set to zero
 __global__ void k_zero_lower(double* X, int n) {
@@ -134,31 +104,181 @@ __global__ void k_sp2_update(double* X, const double* X2, size_t nn, const doubl
     X[e] = sq ? X2[e] : 2.0 * X[e] - X2[e];
 }
 
-int read_info() {
+}  // namespace
+
+namespace sqla {
+
+namespace {
+int read_info(Work& w) {
   rocblas_int info = 0;
-  if (hipMemcpy(&info, g_w.info, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (hipMemcpyAsync(&info, w.info, sizeof(info), hipMemcpyDeviceToHost, w.st) != hipSuccess ||
+      hipStreamSynchronize(w.st) != hipSuccess)
+    return -3;
   return info != 0 ? -5 : 0;
 }
 
-// X = U^-1 with S = U^T U (upper Cholesky factor), unless S is the cached metric
-int set_metric(int n, const double* S) {
+unsigned grid_nn(size_t nn) { return (unsigned)std::min<size_t>((nn + 255) / 256, 4096); }
+
+int gemm(Work& w, rocblas_operation ta, rocblas_operation tb, int n, int k, const double* a, const double* b,
+         double* c) {
+  const double one = 1.0, zero = 0.0;
+  return rocblas_dgemm(w.h, ta, tb, n, n, k, &one, a, n, b, n, &zero, c, n) == rocblas_status_success ? 0 : -4;
+}
+}  // namespace
+
+int ensure(Work& w, int n, hipStream_t st) {
+  if (!w.st) {
+    if (st) {
+      w.st = st;
+    } else {
+      if (hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking) != hipSuccess) return -1;
+      w.own_stream = true;
+    }
+  }
+  if (!w.h) {
+    if (rocblas_create_handle(&w.h) != rocblas_status_success) return -1;
+    if (rocblas_set_stream(w.h, w.st) != rocblas_status_success) return -1;
+  }
+  if (w.n >= n) return 0;
+  if (w.A) {
+    (void)hipStreamSynchronize(w.st);
+    (void)hipFree(w.A); (void)hipFree(w.B); (void)hipFree(w.W); (void)hipFree(w.E);
+    (void)hipFree(w.P); (void)hipFree(w.X); (void)hipFree(w.info); (void)hipFree(w.red);
+    w.A = w.B = w.W = w.E = w.P = w.X = w.red = nullptr;
+    w.info = nullptr;
+  }
+  w.metric_n = 0;
   const size_t nn = (size_t)n * n;
-  if (g_w.metric_n == n && std::memcmp(g_w.metric.data(), S, nn * sizeof(double)) == 0) return 0;
-  g_w.metric_n = 0;
-  if (hipMemcpy(g_w.X, S, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  if (rocsolver_dpotrf(g_w.h, rocblas_fill_upper, n, g_w.X, n, g_w.info) != rocblas_status_success) return -4;
-  if (int rc = read_info()) return rc;
-  if (rocsolver_dtrtri(g_w.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, g_w.X, n, g_w.info) !=
-      rocblas_status_success)
-    return -4;
-  if (int rc = read_info()) return rc;
-  hipLaunchKernelGGL(k_zero_lower, dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)), dim3(256), 0, 0, g_w.X, n);
-  if (hipGetLastError() != hipSuccess) return -3;
-  g_w.metric.assign(S, S + nn);
-  g_w.metric_n = n;
+  if (hipMalloc(&w.A, nn * 8) != hipSuccess || hipMalloc(&w.B, nn * 8) != hipSuccess ||
+      hipMalloc(&w.P, nn * 8) != hipSuccess || hipMalloc(&w.X, nn * 8) != hipSuccess ||
+      hipMalloc(&w.W, (size_t)n * 8) != hipSuccess || hipMalloc(&w.E, (size_t)n * 8) != hipSuccess ||
+      hipMalloc(&w.info, sizeof(rocblas_int)) != hipSuccess || hipMalloc(&w.red, 4 * sizeof(double)) != hipSuccess) {
+    w.n = 0;
+    return -1;
+  }
+  w.n = n;
   return 0;
 }
 
+void release(Work& w) {
+  if (w.st) (void)hipStreamSynchronize(w.st);
+  void* p[] = {w.A, w.B, w.P, w.X, w.W, w.E, w.red, w.info};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  if (w.h) (void)rocblas_destroy_handle(w.h);
+  if (w.own_stream && w.st) (void)hipStreamDestroy(w.st);
+  w = Work{};
+}
+
+int set_metric(Work& w, int n, const double* dS) {
+  const size_t nn = (size_t)n * n;
+  w.metric_n = 0;
+  if (hipMemcpyAsync(w.X, dS, nn * 8, hipMemcpyDeviceToDevice, w.st) != hipSuccess) return -3;
+  if (rocsolver_dpotrf(w.h, rocblas_fill_upper, n, w.X, n, w.info) != rocblas_status_success) return -4;
+  if (int rc = read_info(w)) return rc;
+  if (rocsolver_dtrtri(w.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, w.X, n, w.info) !=
+      rocblas_status_success)
+    return -4;
+  if (int rc = read_info(w)) return rc;
+  hipLaunchKernelGGL(k_zero_lower, dim3(grid_nn(nn)), dim3(256), 0, w.st, w.X, n);
+  if (hipGetLastError() != hipSuccess) return -3;
+  w.metric_n = n;
+  return 0;
+}
+
+int eigen(Work& w, int n, const double* dF, double* de, double* dC) {
+  if (w.metric_n != n) return -1;
+  // A' = X^T F X, dsyevd, C = X C'
+  if (int rc = gemm(w, rocblas_operation_none, rocblas_operation_none, n, n, dF, w.X, w.P)) return rc;
+  if (int rc = gemm(w, rocblas_operation_transpose, rocblas_operation_none, n, n, w.X, w.P, w.A)) return rc;
+  if (rocsolver_dsyevd(w.h, rocblas_evect_original, rocblas_fill_upper, n, w.A, n, de, w.E, w.info) !=
+      rocblas_status_success)
+    return -4;
+  if (int rc = read_info(w)) return rc;
+  return gemm(w, rocblas_operation_none, rocblas_operation_none, n, n, w.X, w.A, dC);
+}
+
+int density(Work& w, int n, int nOcc, const double* dC, double* dP) {
+  if (nOcc == 0) return hipMemsetAsync(dP, 0, (size_t)n * n * 8, w.st) == hipSuccess ? 0 : -3;
+  // column-major C: orbital k in column k, so P = C_occ C_occ^T
+  return gemm(w, rocblas_operation_none, rocblas_operation_transpose, n, nOcc, dC, dC, dP);
+}
+
+int density_sp2(Work& w, int n, int nOcc, const double* dF, double* dP) {
+  const size_t nn = (size_t)n * n;
+  if (nOcc == 0) return hipMemsetAsync(dP, 0, nn * 8, w.st) == hipSuccess ? 0 : -3;
+  if (nOcc == n) return 1;  // full occupation: P = S^-1, left to the eigenvector path
+  if (w.metric_n != n) return -1;
+  if (int rc = gemm(w, rocblas_operation_none, rocblas_operation_none, n, n, dF, w.X, w.P)) return rc;
+  if (int rc = gemm(w, rocblas_operation_transpose, rocblas_operation_none, n, n, w.X, w.P, w.A)) return rc;
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(kRedBlock), 0, w.st, w.A, n, w.red);
+  hipLaunchKernelGGL(k_sp2_init, dim3(grid_nn(nn)), dim3(256), 0, w.st, w.P, w.A, n, w.red);
+  if (hipGetLastError() != hipSuccess) return -3;
+  // P holds the iterate, B its square.  The idempotency error tr X - tr X^2
+  // of the iterate entering a step is read every 8 steps; once below 1e-11
+  // four more steps take it to rounding level (each step squares it).
+  constexpr int kMaxSteps = 96, kCheck = 8;
+  int extra = -1;
+  double red[4] = {0, 0, 0, 0};
+  auto square = [&]() -> int {
+    if (int rc = gemm(w, rocblas_operation_none, rocblas_operation_none, n, n, w.P, w.P, w.B)) return rc;
+    hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, w.st, w.P, w.B, n, w.red);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+  };
+  auto read_red = [&]() -> int {
+    if (hipMemcpyAsync(red, w.red, sizeof(red), hipMemcpyDeviceToHost, w.st) != hipSuccess ||
+        hipStreamSynchronize(w.st) != hipSuccess)
+      return -3;
+    return 0;
+  };
+  for (int it = 0; it < kMaxSteps; ++it) {
+    if (int rc = square()) return rc;
+    if (extra < 0 && (it + 1) % kCheck == 0) {
+      if (int rc = read_red()) return rc;
+      if (!(red[2] == red[2]) || !(red[3] == red[3])) return 1;  // NaN
+      // a spectrum of rounding-level width has no meaningful lowest nOcc
+      if (!(red[1] - red[0] > 1e-10 * std::max(1.0, std::fabs(red[1])))) return 1;
+      if (std::fabs(red[2] - red[3]) < 1e-11) extra = 4;
+    }
+    hipLaunchKernelGGL(k_sp2_update, dim3(grid_nn(nn)), dim3(256), 0, w.st, w.P, w.B, nn, w.red, nOcc);
+    if (hipGetLastError() != hipSuccess) return -3;
+    if (extra > 0 && --extra == 0) break;
+  }
+  if (extra != 0) return 1;
+  if (int rc = square()) return rc;
+  if (int rc = read_red()) return rc;
+  if (!(std::fabs(red[2] - red[3]) < 1e-11) || !(std::fabs(red[2] - nOcc) < 1e-8)) return 1;
+  // P_AO = X P' X^T
+  if (int rc = gemm(w, rocblas_operation_none, rocblas_operation_none, n, n, w.X, w.P, w.B)) return rc;
+  return gemm(w, rocblas_operation_none, rocblas_operation_transpose, n, n, w.B, w.X, dP);
+}
+
+}  // namespace sqla
+
+// ------------------------------------------------- host-array entry points
+namespace {
+std::mutex g_mu;
+sqla::Work g_w;                   // shared by the host-array entry points
+std::vector<double> g_metric;     // host copy of the metric factored in g_w
+
+// device copies of host inputs, metric from a host S (cached while S repeats)
+int host_setup(int n, const double* S) {
+  if (sqla::ensure(g_w, n)) return -2;
+  const size_t nn = (size_t)n * n;
+  if (g_w.metric_n == n && g_metric.size() == nn && std::memcmp(g_metric.data(), S, nn * sizeof(double)) == 0)
+    return 0;
+  if (hipMemcpyAsync(g_w.B, S, nn * 8, hipMemcpyHostToDevice, g_w.st) != hipSuccess) return -3;
+  if (int rc = sqla::set_metric(g_w, n, g_w.B)) return rc;
+  g_metric.assign(S, S + nn);
+  return 0;
+}
+
+int download(void* dst, const void* src, size_t bytes) {
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_w.st) != hipSuccess ||
+      hipStreamSynchronize(g_w.st) != hipSuccess)
+    return -3;
+  return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -166,33 +286,16 @@ extern "C" {
 int sqla_gen_eigen(int n, const double* F, const double* S, double* e, double* C) {
   if (n < 1) return -1;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (ensure(n)) return -2;
-  if (int rc = set_metric(n, S)) return rc;
+  if (int rc = host_setup(n, S)) return rc;
   const size_t nn = (size_t)n * n;
-  const double one = 1.0, zero = 0.0;
-  // symmetric input: row-major == column-major.  A' = X^T F X
-  if (hipMemcpy(g_w.A, F, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.A, n, g_w.X, n,
-                    &zero, g_w.P, n) != rocblas_status_success ||
-      rocblas_dgemm(g_w.h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
-                    &zero, g_w.A, n) != rocblas_status_success)
-    return -4;
-  if (rocsolver_dsyevd(g_w.h, rocblas_evect_original, rocblas_fill_upper, n, g_w.A, n, g_w.W, g_w.E, g_w.info) !=
-      rocblas_status_success)
-    return -4;
-  if (int rc = read_info()) return rc;
-  // C = X C'; column k (column-major) is orbital k: C[k*n + i]
-  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.A, n,
-                    &zero, g_w.P, n) != rocblas_status_success)
-    return -4;
-  if (hipMemcpy(C, g_w.P, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (hipMemcpy(e, g_w.W, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  return 0;
+  // F is staged in B; eigen() reads it once, before it writes C into B
+  if (hipMemcpyAsync(g_w.B, F, nn * 8, hipMemcpyHostToDevice, g_w.st) != hipSuccess) return -3;
+  int rc = sqla::eigen(g_w, n, g_w.B, g_w.W, g_w.B);
+  if (!rc) rc = download(C, g_w.B, nn * 8);
+  if (!rc) rc = download(e, g_w.W, (size_t)n * 8);
+  return rc;
 }
 
-// P = sum_{k < nOcc} C_k C_k^T with C orbital-major (C[k*n + i]); one DGEMM
-// on the device (matrix cores).  Read as column-major, C is n x n with
-// orbital k in column k, so P = C_occ C_occ^T.
 int sqla_density(int n, int nOcc, const double* C, double* P) {
   if (n < 1 || nOcc < 0 || nOcc > n) return -1;
   const size_t nn = (size_t)n * n;
@@ -201,14 +304,10 @@ int sqla_density(int n, int nOcc, const double* C, double* P) {
     return 0;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  if (ensure(n)) return -2;
-  if (hipMemcpy(g_w.A, C, (size_t)nOcc * n * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  const double one = 1.0, zero = 0.0;
-  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_transpose, n, n, nOcc, &one, g_w.A, n,
-                    g_w.A, n, &zero, g_w.P, n) != rocblas_status_success)
-    return -4;
-  if (hipMemcpy(P, g_w.P, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  return 0;
+  if (sqla::ensure(g_w, n)) return -2;
+  if (hipMemcpyAsync(g_w.A, C, (size_t)nOcc * n * 8, hipMemcpyHostToDevice, g_w.st) != hipSuccess) return -3;
+  if (int rc = sqla::density(g_w, n, nOcc, g_w.A, g_w.B)) return rc;
+  return download(P, g_w.B, nn * 8);
 }
 
 // Density P = sum_{k < nOcc} C_k C_k^T of the lowest nOcc solutions of
@@ -225,60 +324,16 @@ int sqla_density_sp2(int n, int nOcc, const double* F, const double* S, double* 
   const size_t nn = (size_t)n * n;
   if (nOcc == 0 || nOcc == n) {
     if (nOcc == 0) { std::memset(P, 0, nn * sizeof(double)); return 0; }
-    return 1;  // full occupation: P = S^-1, left to the eigenvector path
+    return 1;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  if (ensure(n)) return -2;
-  if (int rc = set_metric(n, S)) return rc;
-  const double one = 1.0, zero = 0.0;
-  if (hipMemcpy(g_w.A, F, nn * 8, hipMemcpyHostToDevice) != hipSuccess) return -3;
-  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.A, n, g_w.X, n,
-                    &zero, g_w.P, n) != rocblas_status_success ||
-      rocblas_dgemm(g_w.h, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
-                    &zero, g_w.A, n) != rocblas_status_success)
-    return -4;
-  const unsigned grid = (unsigned)std::min<size_t>((nn + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(kRedBlock), 0, 0, g_w.A, n, g_w.red);
-  hipLaunchKernelGGL(k_sp2_init, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.A, n, g_w.red);
-  if (hipGetLastError() != hipSuccess) return -3;
-  // P holds the iterate, B its square.  The idempotency error tr X - tr X^2
-  // of the iterate entering a step is read every 8 steps; once below 1e-11
-  // four more steps take it to rounding level (each step squares it).
-  constexpr int kMaxSteps = 96, kCheck = 8;
-  int extra = -1;
-  double red[4] = {0, 0, 0, 0};
-  auto square = [&]() -> int {
-    if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.P, n, g_w.P, n,
-                      &zero, g_w.B, n) != rocblas_status_success)
-      return -4;
-    hipLaunchKernelGGL(k_sp2_trace, dim3(1), dim3(kRedBlock), 0, 0, g_w.P, g_w.B, n, g_w.red);
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-  };
-  for (int it = 0; it < kMaxSteps; ++it) {
-    if (int rc = square()) return rc;
-    if (extra < 0 && (it + 1) % kCheck == 0) {
-      if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
-      if (!(red[2] == red[2]) || !(red[3] == red[3])) return 1;  // NaN
-      // a spectrum of rounding-level width has no meaningful lowest nOcc
-      if (!(red[1] - red[0] > 1e-10 * std::max(1.0, std::fabs(red[1])))) return 1;
-      if (std::fabs(red[2] - red[3]) < 1e-11) extra = 4;
-    }
-    hipLaunchKernelGGL(k_sp2_update, dim3(grid), dim3(256), 0, 0, g_w.P, g_w.B, nn, g_w.red, nOcc);
-    if (hipGetLastError() != hipSuccess) return -3;
-    if (extra > 0 && --extra == 0) break;
-  }
-  if (extra != 0) return 1;
-  if (int rc = square()) return rc;
-  if (hipMemcpy(red, g_w.red, sizeof(red), hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (!(std::fabs(red[2] - red[3]) < 1e-11) || !(std::fabs(red[2] - nOcc) < 1e-8)) return 1;
-  // P_AO = X P' X^T
-  if (rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, g_w.X, n, g_w.P, n,
-                    &zero, g_w.B, n) != rocblas_status_success ||
-      rocblas_dgemm(g_w.h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, g_w.B, n, g_w.X, n,
-                    &zero, g_w.A, n) != rocblas_status_success)
-    return -4;
-  if (hipMemcpy(P, g_w.A, nn * 8, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  return 0;
+  if (int rc = host_setup(n, S)) return rc;
+  // F is staged in B (read once, before B holds the squares); the result
+  // lands in A (free after the first steps)
+  if (hipMemcpyAsync(g_w.B, F, nn * 8, hipMemcpyHostToDevice, g_w.st) != hipSuccess) return -3;
+  int rc = sqla::density_sp2(g_w, n, nOcc, g_w.B, g_w.A);
+  if (rc == 0) rc = download(P, g_w.A, nn * 8);
+  return rc;
 }
 
 int gen_sym_eigen(int nDim, const double* A, const double* S, double* e, double* C) {

@@ -9,7 +9,9 @@
  *   eigenpairs; E = E_NN + sum (H + J/2)(PA+PB) + 1/2 sum (KA PA + KB PB);
  *   density DIIS (four points after three damped steps); converged when
  *   |dE| <= SCFConv and rms dP <= SCFConv; 0.0 returned if not converged.
- * The J/K builds (uhf_getGMatrix) and the eigen-solves run on the device.
+ * The guess and the one-electron set-up are prepared here; the iteration
+ * itself runs with all of its matrices on the device (sqscf_*, in
+ * csrc/sq_scf_dev.hip), any opt->convMethod (damping, DIIS2/3/4).
  * The one difference from the reference is that every J/K build uses exact
  * integrals: the reference's monopole substitution for far quartets
  * (src/matrix.c:3946-3949) is not applied.
@@ -61,111 +63,145 @@ static void normalize_orbitals(int n, int nOcc, const double *S, double *C)
 }
 
 /* ---------------------------------------------------------------- DIIS */
-static struct {
+/* Density damping and K-point density DIIS (src/conv.c:40-844).  Each
+ * method keeps its own history, as the reference keeps its own statics:
+ * inputs P[0..K-1] (P[K-1] the newest stored density) and errors
+ * dP[0..K-1].  A call stores the error cur - P[K-1] in dP[K-1], damps
+ * (the first K-1 calls, every call for K = 1) or extrapolates with
+ * P[a] + dP[a], a < K-1, and cur, then shifts both lists by one and stores
+ * the result as the newest input. */
+struct diis_hist {
 	int it, n;
-	double *P[4][2], *dP[4][2];   /* slot 3 = latest stored input */
-} g_diis;
+	double *P[4][2], *dP[4][2];
+};
+static struct diis_hist g_damp, g_diis2, g_diis3, g_diis4;
 
-static void diis_free(void)
+static void diis_free(struct diis_hist *h)
 {
 	for (int s = 0; s < 4; s++)
 		for (int k = 0; k < 2; k++) {
-			free(g_diis.P[s][k]);
-			free(g_diis.dP[s][k]);
-			g_diis.P[s][k] = g_diis.dP[s][k] = NULL;
+			free(h->P[s][k]);
+			free(h->dP[s][k]);
+			h->P[s][k] = h->dP[s][k] = NULL;
 		}
-	g_diis.it = 0;
-	g_diis.n = 0;
+	h->it = 0;
+	h->n = 0;
 }
 
-double conv_diis4(int nBasis, double alpha, double *PA, double *PB)
+static double conv_density(struct diis_hist *h, int K, const char *name, int nBasis, double alpha, double *PA,
+                           double *PB)
 {
 	if (nBasis == 0 && alpha == 0.0 && PA == NULL && PB == NULL) {
-		diis_free();
+		diis_free(h);
 		return 0.0;
 	}
 	if (alpha < 0.0 || alpha > 1.0) {
-		printf("conv_diis4 : invalid range of alpha\n");
+		printf("%s : invalid range of alpha\n", name);
 		exit(-1);
 	}
 	const size_t nn = (size_t)nBasis * nBasis;
 	double *cur[2] = {PA, PB};
-	if (g_diis.it == 0) {
-		if (g_diis.P[0][0]) {
-			printf("conv_diis4 : detect prior initialized pointers\n");
+	if (h->it == 0) {
+		if (h->P[0][0]) {
+			printf("%s : detect prior initialized pointers\n", name);
 			exit(-1);
 		}
-		g_diis.n = nBasis;
-		for (int s = 0; s < 4; s++)
+		h->n = nBasis;
+		for (int s = 0; s < K; s++)
 			for (int k = 0; k < 2; k++) {
-				g_diis.P[s][k] = calloc(nn, sizeof(double));
-				g_diis.dP[s][k] = calloc(nn, sizeof(double));
-				if (!g_diis.P[s][k] || !g_diis.dP[s][k]) {
-					printf("conv_diis4 : cannot allocate memory\n");
+				h->P[s][k] = calloc(nn, sizeof(double));
+				h->dP[s][k] = calloc(nn, sizeof(double));
+				if (!h->P[s][k] || !h->dP[s][k]) {
+					printf("%s : cannot allocate memory\n", name);
 					exit(-1);
 				}
 			}
-		for (int k = 0; k < 2; k++) memcpy(g_diis.P[3][k], cur[k], nn * sizeof(double));
-		g_diis.it = 1;
+		for (int k = 0; k < 2; k++) memcpy(h->P[K - 1][k], cur[k], nn * sizeof(double));
+		h->it = 1;
 		return 0.0;
 	}
 	double rms = 0.0;
 	for (int k = 0; k < 2; k++)
 		for (size_t e = 0; e < nn; e++) {
-			const double t = cur[k][e] - g_diis.P[3][k][e];
+			const double t = cur[k][e] - h->P[K - 1][k][e];
 			rms += t * t;
-			g_diis.dP[3][k][e] = t;
+			h->dP[K - 1][k][e] = t;
 		}
 	rms = sqrt(rms / nBasis / nBasis / 2.0);
-	if (g_diis.it <= 3) {
+	if (K == 1 || h->it <= K - 1) {
 		for (int k = 0; k < 2; k++)
-			for (size_t e = 0; e < nn; e++) cur[k][e] = alpha * cur[k][e] + (1.0 - alpha) * g_diis.P[3][k][e];
+			for (size_t e = 0; e < nn; e++) cur[k][e] = alpha * cur[k][e] + (1.0 - alpha) * h->P[K - 1][k][e];
 	} else {
 		double B[4][4];
-		for (int a = 0; a < 4; a++)
-			for (int b = a; b < 4; b++) {
+		for (int a = 0; a < K; a++)
+			for (int b = a; b < K; b++) {
 				double s = 0.0;
 				for (int k = 0; k < 2; k++)
-					for (size_t e = 0; e < nn; e++) s += g_diis.dP[a][k][e] * g_diis.dP[b][k][e];
+					for (size_t e = 0; e < nn; e++) s += h->dP[a][k][e] * h->dP[b][k][e];
 				B[a][b] = B[b][a] = s;
 			}
-		double A[25], rhs[5] = {0, 0, 0, 0, -1};
-		for (int a = 0; a < 5; a++)
-			for (int b = 0; b < 5; b++)
-				A[a * 5 + b] = (a < 4 && b < 4) ? B[a][b] : ((a == 4 && b == 4) ? 0.0 : -1.0);
-		linear_solver(5, 1, A, rhs);
-		double c[4] = {rhs[0], rhs[1], rhs[2], rhs[3]};
-		if (B[3][3] > B[2][2]) { c[3] = alpha; c[2] = 1.0 - alpha; c[1] = 0.0; c[0] = 0.0; }
+		double c[4] = {0, 0, 0, 0};
+		if (K == 2) {  /* src/conv.c:243-249 */
+			double c1 = (B[0][0] - B[0][1]) / (B[1][1] - B[0][1]);
+			double c0 = 1.0 / (1.0 + c1);
+			c1 *= c0;
+			if (B[1][1] > B[0][0]) { c1 = alpha; c0 = 1.0 - alpha; }
+			c[0] = c0; c[1] = c1;
+		} else if (K == 3) {  /* src/conv.c:413-428 */
+			const double B00 = B[0][0], B11 = B[1][1], B22 = B[2][2], B01 = B[0][1], B02 = B[0][2], B12 = B[1][2];
+			const double den = B12 * B12 - B01 * B12 + B02 * B11 + B01 * B22 - B02 * B12 - B11 * B22;
+			double c1 = (B02 * B02 + B00 * B12 - B01 * B02 - B00 * B22 + B01 * B22 - B02 * B12) / den;
+			double c2 = (B01 * B01 - B00 * B11 - B01 * B02 + B02 * B11 + B12 * B00 - B12 * B01) / den;
+			double c0 = 1.0 / (1.0 + c1 + c2);
+			c1 *= c0;
+			c2 *= c0;
+			if (B22 > B11) { c2 = alpha; c1 = 1.0 - alpha; c0 = 0.0; }
+			c[0] = c0; c[1] = c1; c[2] = c2;
+		} else {  /* src/conv.c:775-795 */
+			double A[25], rhs[5] = {0, 0, 0, 0, -1};
+			for (int a = 0; a < 5; a++)
+				for (int b = 0; b < 5; b++)
+					A[a * 5 + b] = (a < 4 && b < 4) ? B[a][b] : ((a == 4 && b == 4) ? 0.0 : -1.0);
+			linear_solver(5, 1, A, rhs);
+			c[0] = rhs[0]; c[1] = rhs[1]; c[2] = rhs[2]; c[3] = rhs[3];
+			if (B[3][3] > B[2][2]) { c[3] = alpha; c[2] = 1.0 - alpha; c[1] = 0.0; c[0] = 0.0; }
+		}
 		for (int k = 0; k < 2; k++)
-			for (size_t e = 0; e < nn; e++)
-				cur[k][e] = c[3] * cur[k][e] + c[2] * (g_diis.P[2][k][e] + g_diis.dP[2][k][e]) +
-				            c[1] * (g_diis.P[1][k][e] + g_diis.dP[1][k][e]) +
-				            c[0] * (g_diis.P[0][k][e] + g_diis.dP[0][k][e]);
+			for (size_t e = 0; e < nn; e++) {
+				double v = c[K - 1] * cur[k][e];
+				for (int a = 0; a < K - 1; a++) v += c[a] * (h->P[a][k][e] + h->dP[a][k][e]);
+				cur[k][e] = v;
+			}
 	}
 	for (int k = 0; k < 2; k++) {
-		double *t = g_diis.dP[0][k];
-		g_diis.dP[0][k] = g_diis.dP[1][k]; g_diis.dP[1][k] = g_diis.dP[2][k]; g_diis.dP[2][k] = g_diis.dP[3][k];
-		g_diis.dP[3][k] = t;
-		double *u = g_diis.P[0][k];
-		g_diis.P[0][k] = g_diis.P[1][k]; g_diis.P[1][k] = g_diis.P[2][k]; g_diis.P[2][k] = g_diis.P[3][k];
-		g_diis.P[3][k] = u;
-		memcpy(g_diis.P[3][k], cur[k], nn * sizeof(double));
+		double *t = h->dP[0][k], *u = h->P[0][k];
+		for (int s = 0; s + 1 < K; s++) { h->dP[s][k] = h->dP[s + 1][k]; h->P[s][k] = h->P[s + 1][k]; }
+		h->dP[K - 1][k] = t;
+		h->P[K - 1][k] = u;
+		memcpy(h->P[K - 1][k], cur[k], nn * sizeof(double));
 	}
-	g_diis.it++;
+	h->it++;
 	return rms;
 }
 
-/* --------------------------------------------------------------- energy */
-static double total_energy(int n, const struct Molecule_t *mol, const struct option_t *opt, const double *PA,
-                           const double *PB, const double *H, const double *JT, const double *KA, const double *KB)
+double conv_damping(int nBasis, double alpha, double *PA, double *PB)
 {
-	double E = 0.0;
-	for (int i = 0; i < mol->nAtom; i++)
-		E -= mol->Z[i] * (mol->x[i] * opt->Ex + mol->y[i] * opt->Ey + mol->z[i] * opt->Ez);
-	E += nuclei_coulomb(mol);
-	for (size_t e = 0; e < (size_t)n * n; e++)
-		E += (H[e] + 0.5 * JT[e]) * (PA[e] + PB[e]) + 0.5 * KA[e] * PA[e] + 0.5 * KB[e] * PB[e];
-	return E;
+	return conv_density(&g_damp, 1, "conv_damping", nBasis, alpha, PA, PB);
+}
+
+double conv_diis2(int nBasis, double alpha, double *PA, double *PB)
+{
+	return conv_density(&g_diis2, 2, "conv_diis2", nBasis, alpha, PA, PB);
+}
+
+double conv_diis3(int nBasis, double alpha, double *PA, double *PB)
+{
+	return conv_density(&g_diis3, 3, "conv_diis3", nBasis, alpha, PA, PB);
+}
+
+double conv_diis4(int nBasis, double alpha, double *PA, double *PB)
+{
+	return conv_density(&g_diis4, 4, "conv_diis4", nBasis, alpha, PA, PB);
 }
 
 static double now_s(void)
@@ -197,15 +233,6 @@ static void one_electron(int n, const struct GTOBasis_t *gto, const struct Molec
 				                GTO_moment(i, j, gto, 0, 0, 1, 0.0, 0.0, 0.0) * opt->Ez;
 }
 
-static void density_or_die(int n, int nOcc, const double *C, double *P)
-{
-	int rc = sqla_density(n, nOcc, C, P);
-	if (rc) {
-		printf("uhf: Error - device density build failed (%d)\n", rc);
-		exit(EXIT_FAILURE);
-	}
-}
-
 static void eigen_or_die(int n, int nOcc, const double *F, const double *S, double *e, double *C)
 {
 	if (nOcc == 0) return;
@@ -216,23 +243,24 @@ static void eigen_or_die(int n, int nOcc, const double *F, const double *S, doub
 	}
 }
 
-/* The density of the lowest nOcc solutions of F C = S C e.  With use_proj it
- * comes straight from F (spectral projection, no orbitals; e and C are not
- * updated); otherwise, or when the projection does not settle (no gap at
- * nOcc), from the orbitals as in the reference (src/uhf.c:1703-1710). */
-static void density_from_fock(int n, int nOcc, const double *F, const double *S, double *e, double *C,
-                              double *P, int use_proj)
+/* -SAVECHECKALL: the checkpoint of every iteration (src/uhf.c:1739-1741) */
+struct save_each {
+	int n;
+	struct GTOBasis_t *gto;
+	struct Molecule_t *mol;
+	struct option_t *opt;
+	double *CA, *CB, *eA, *eB;
+};
+
+static void save_each_iter(void *user, sqscf_ctx *ctx, int iter, double E, double dE, double rmsDP)
 {
-	if (use_proj) {
-		int rc = sqla_density_sp2(n, nOcc, F, S, P);
-		if (rc == 0) return;
-		if (rc < 0) {
-			printf("uhf: Error - device density projection failed (%d)\n", rc);
-			exit(EXIT_FAILURE);
-		}
+	struct save_each *s = user;
+	(void)iter;
+	if (sqscf_orbitals(ctx, s->CA, s->CB, s->eA, s->eB)) {
+		printf("uhf: Error - %s\n", sqscf_errmsg());
+		exit(EXIT_FAILURE);
 	}
-	eigen_or_die(n, nOcc, F, S, e, C);
-	density_or_die(n, nOcc, C, P);
+	save_checkpoint(s->n, s->gto, s->mol, E, dE, rmsDP, s->CA, s->CB, s->eA, s->eB, s->opt);
 }
 
 double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, int nEB, double *CA,
@@ -244,13 +272,15 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 		printf("uhf - error number of electron in each spin is not the same\n");
 		exit(-1);
 	}
-	double *JT, *dJT, *KA, *dKA, *KB, *dKB, *PA, *dPA, *PB, *dPB, *FA, *FB, *H, *S;
-	ALLOC(JT, nn); ALLOC(dJT, nn); ALLOC(KA, nn); ALLOC(dKA, nn); ALLOC(KB, nn); ALLOC(dKB, nn);
-	ALLOC(PA, nn); ALLOC(dPA, nn); ALLOC(PB, nn); ALLOC(dPB, nn); ALLOC(FA, nn); ALLOC(FB, nn);
-	ALLOC(H, nn); ALLOC(S, nn);
+	if (opt->convMethod < CONVMETHOD_DAMPING || opt->convMethod > CONVMETHOD_DIIS4) {
+		printf("uhf - error unknown convergence method %d\n", opt->convMethod);
+		exit(-1);
+	}
+	double *PA, *PB, *H, *S;
+	ALLOC(PA, nn); ALLOC(PB, nn); ALLOC(H, nn); ALLOC(S, nn);
 	/* SQ_SCF_TIMING=1 prints the wall time of each phase to stderr */
 	const int timing = getenv("SQ_SCF_TIMING") != NULL;
-	double tp[7] = {0}, t0 = now_s();
+	double tp[4] = {0}, t0 = now_s();
 	one_electron(n, gto, mol, opt, H, S);
 	tp[0] += now_s() - t0;
 
@@ -294,124 +324,59 @@ double uhf(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int nEA, 
 
 	if (opt->loadDMatrix) sq_load_density(n, opt->restricted, opt->DMatrixFile, PA, PB);  /* src/uhf.c:1414-1462 */
 
-	double sA = 0.0, sB = 0.0;
-	for (size_t e = 0; e < nn; e++) { sA += PA[e]; sB += PB[e]; }
-#define GETCUTOFF(acc) ((acc) / (sA * sA + sA * sB + sB * sB) / 50.0)
-	const double realCutoff = GETCUTOFF(opt->SCFConv);
-	opt->SCFCutoff = realCutoff;
+	/* the SCF loop with its state on the device; one J/K context gives the
+	 * Schwarz bounds and every build */
 	t0 = now_s();
-	double *Schwarz = create_Schwarz(n, gto);
-	tp[1] += now_s() - t0;
-
-	conv_diis4(0, 0.0, NULL, NULL);
-	const double gamma = opt->SCFDrag;
-	conv_diis4(n, gamma, PA, PB);
-	memcpy(dPA, PA, nn * sizeof(double));
-	memcpy(dPB, PB, nn * sizeof(double));
-
-	double cutA = 0.0, cutB = 0.0, cutC = 0.0, thisCutoff;
-	if (opt->SCFAccuracy == SCFACCURACY_3STEP) {
-		cutA = GETCUTOFF(SCFACCURACY_3STEP_A);
-		cutB = GETCUTOFF(SCFACCURACY_3STEP_B);
-		cutC = GETCUTOFF(SCFACCURACY_3STEP_C);
-		thisCutoff = cutA;
-	} else {
-		thisCutoff = realCutoff;
+	sqscf_ctx *ctx = NULL;
+	if (sqscf_create(0, &ctx) || sqscf_set_system(ctx, n, gto, H, S) || sqscf_set_density(ctx, PA, PB)) {
+		printf("uhf: Error - device SCF set-up failed: %s\n", sqscf_errmsg());
+		exit(EXIT_FAILURE);
 	}
-#undef GETCUTOFF
-
-	double Etot = 0.0, dE = 0.0, avgdP = 0.0;
-	int iter = 0, notConverged = 1;
-	do {
-		iter++;
-		t0 = now_s();
-		uhf_getGMatrix(n, gto, Schwarz, thisCutoff, dPA, opt->restricted ? dPA : dPB, dJT, dKA, dKB, opt);
-		tp[2] += now_s() - t0;
-		t0 = now_s();
-		for (size_t e = 0; e < nn; e++) {
-			JT[e] += dJT[e];
-			KA[e] += dKA[e];
-			KB[e] += dKB[e];
-			dPA[e] = PA[e];
-			dPB[e] = PB[e];
-			FA[e] = H[e] + JT[e] + KA[e];
-			FB[e] = H[e] + JT[e] + KB[e];
-		}
-		tp[5] += now_s() - t0;
-		/* The new densities come straight from FA, FB by spectral
-		 * projection; the orbitals and orbital energies are solved once,
-		 * after the loop, from the last FA, FB (or every iteration when
-		 * -SAVECHECKALL writes them).  The device eigen-solver returns
-		 * S-orthonormal orbitals, so the reference's re-normalisation
-		 * (src/uhf.c:1705-1706) is a no-op here.  Restricted: KB = KA, so
-		 * FB = FA and the beta solve would repeat the alpha one exactly. */
-		t0 = now_s();
-		density_from_fock(n, nEA, FA, S, eA, CA, PA, !opt->saveCheckAll);
-		if (opt->restricted && nEA == nEB) {
-			memcpy(PB, PA, nn * sizeof(double));
-			if (opt->saveCheckAll) {
-				memcpy(CB, CA, nn * sizeof(double));
-				memcpy(eB, eA, (size_t)n * sizeof(double));
-			}
-		} else {
-			density_from_fock(n, nEB, FB, S, eB, CB, PB, !opt->saveCheckAll);
-		}
-		tp[4] += now_s() - t0;
-		t0 = now_s();
-		dE = Etot;
-		Etot = total_energy(n, mol, opt, PA, PB, H, JT, KA, KB);
-		dE = Etot - dE;
-		avgdP = conv_diis4(n, gamma, PA, PB);
-		notConverged = fabs(dE) > opt->SCFConv || avgdP > opt->SCFConv;
-		for (size_t e = 0; e < nn; e++) {
-			dPA[e] = PA[e] - dPA[e];
-			dPB[e] = PB[e] - dPB[e];
-		}
-		tp[5] += now_s() - t0;
-		if (opt->saveCheckAll) save_checkpoint(n, gto, mol, Etot, dE, avgdP, CA, CB, eA, eB, opt);
-		if (iter >= opt->SCFMax) break;
-		if (opt->SCFAccuracy == SCFACCURACY_3STEP) {
-			if (!notConverged) break;
-			if ((thisCutoff == cutA && fabs(dE) <= SCFACCURACY_3STEP_A && avgdP <= SCFACCURACY_3STEP_A) ||
-			    (thisCutoff == cutB && fabs(dE) <= SCFACCURACY_3STEP_B && avgdP <= SCFACCURACY_3STEP_B) ||
-			    (thisCutoff == cutC && fabs(dE) <= SCFACCURACY_3STEP_C && avgdP <= SCFACCURACY_3STEP_C)) {
-				if (thisCutoff == cutA) thisCutoff = cutB;
-				else if (thisCutoff == cutB) thisCutoff = cutC;
-				else if (thisCutoff == cutC) thisCutoff = realCutoff;
-			}
-		}
-	} while (notConverged);
-
+	tp[1] += now_s() - t0;
+	double Econst = nuclei_coulomb(mol);
+	for (int i = 0; i < mol->nAtom; i++)
+		Econst -= mol->Z[i] * (mol->x[i] * opt->Ex + mol->y[i] * opt->Ey + mol->z[i] * opt->Ez);
+	struct save_each se = {n, gto, mol, opt, CA, CB, eA, eB};
+	sqscf_params prm;
+	memset(&prm, 0, sizeof(prm));
+	prm.restricted = opt->restricted;
+	prm.nEA = nEA;
+	prm.nEB = nEB;
+	prm.SCFConv = opt->SCFConv;
+	prm.SCFMax = opt->SCFMax;
+	prm.SCFDrag = opt->SCFDrag;
+	prm.threeStep = opt->SCFAccuracy == SCFACCURACY_3STEP;
+	prm.convMethod = opt->convMethod;
+	prm.Econst = Econst;
+	prm.mixedThreshold = sqjk_get_dropin_mixed();
+	prm.eigenEveryIter = opt->saveCheckAll;
+	prm.onIter = opt->saveCheckAll ? save_each_iter : NULL;
+	prm.user = &se;
+	sqscf_result res;
+	t0 = now_s();
+	if (sqscf_run(ctx, &prm, &res)) {
+		printf("uhf: Error - device SCF failed: %s\n", sqscf_errmsg());
+		exit(EXIT_FAILURE);
+	}
+	tp[2] += now_s() - t0;
+	opt->SCFCutoff = res.realCutoff;
 	/* orbitals and orbital energies of the last Fock matrices */
 	t0 = now_s();
-	eigen_or_die(n, nEA, FA, S, eA, CA);
-	if (opt->restricted && nEA == nEB) {
-		memcpy(CB, CA, nn * sizeof(double));
-		memcpy(eB, eA, (size_t)n * sizeof(double));
-	} else {
-		eigen_or_die(n, nEB, FB, S, eB, CB);
+	if (sqscf_orbitals(ctx, CA, CB, eA, eB) || sqscf_density(ctx, PA, PB)) {
+		printf("uhf: Error - %s\n", sqscf_errmsg());
+		exit(EXIT_FAILURE);
 	}
 	tp[3] += now_s() - t0;
+	sqscf_destroy(ctx);
 
 	/* density and checkpoint files (src/uhf.c:1843-1883) */
 	if (opt->saveDMatrix) sq_save_density(n, opt->DMatrixFile, PA, PB);
-	if (opt->saveCheck) save_checkpoint(n, gto, mol, Etot, dE, avgdP, CA, CB, eA, eB, opt);
-
-	/* release the J/K state (the reference resets its statics here) */
-	{
-		double z = 0.0;
-		t0 = now_s();
-		GTO_JK_Matrix_Quartet_Parallel(0, 0, &z, &z, gto, &z, 0.0, &z, &z, &z, opt);
-		tp[6] += now_s() - t0;
-	}
+	if (opt->saveCheck) save_checkpoint(n, gto, mol, res.energy, res.dE, res.rmsDP, CA, CB, eA, eB, opt);
 	if (timing)
-		fprintf(stderr, "uhf timing [s]: one-electron %.3f  Schwarz %.3f  J/K %.3f  eigen %.3f  density %.3f"
-		        "  host %.3f  release %.3f  (%d iterations)\n", tp[0], tp[1], tp[2], tp[3], tp[4], tp[5], tp[6], iter);
-	conv_diis4(0, 0.0, NULL, NULL);
-	free(Schwarz);
-	free(JT); free(dJT); free(KA); free(dKA); free(KB); free(dKB);
-	free(PA); free(dPA); free(PB); free(dPB); free(FA); free(FB); free(H); free(S);
-	return notConverged ? 0.0 : Etot;
+		fprintf(stderr, "uhf timing [s]: one-electron %.3f  set-up (Schwarz, metric) %.3f  SCF loop %.3f"
+		        "  orbitals %.3f  (%d iterations)\n", tp[0], tp[1], tp[2], tp[3], res.iterations);
+	free(PA); free(PB); free(H); free(S);
+	return res.converged ? res.energy : 0.0;
 }
 
 /* Legacy RHF (src/rhf.c:286-548): core-Hamiltonian guess, G = J[P] - K[P]/2

@@ -62,7 +62,7 @@ class Option(C.Structure):
     ]
 
 
-CONVMETHOD_DIIS4 = 3
+CONVMETHOD_DAMPING, CONVMETHOD_DIIS2, CONVMETHOD_DIIS3, CONVMETHOD_DIIS4 = 0, 1, 2, 3
 SCFACCURACY_3STEP, SCFACCURACY_1STEP = 0, 1
 SCFGUESS_DIAG, SCFGUESS_CORE, SCFGUESS_CHECK, SCFGUESS_CACB = 0, 1, 2, 3
 
