@@ -161,6 +161,60 @@ def cpu_baseline(bas, P, cut, slices):
                       f"x{slices}"}
 
 
+def batch_bench(steps, mixed_thr, cut=1e-10):
+    """Config C4: 256 seeded small molecules (a quarter UHF doublet cations)
+    as one graph-captured set of class launches, against the same builds
+    one molecule at a time through the plain path (sqjk_build, FP64, host
+    arrays in and out, as uhf_getGMatrix does)."""
+    from sqamd import tune
+    inst, bases, engines, batch, dens = tune.make_batch(cut=cut)
+    out = {"workload": f"{len(inst)} seeded molecules (H2O..C6H6, STO-3G/6-31G/6-31G*, "
+                       f"{min(b.n for b in bases)}-{max(b.n for b in bases)} functions, "
+                       f"{sum(not x['restricted'] for x in inst)} UHF doublet cations, sigma 0.05 A), "
+                       f"seeded densities, cut-off {cut:g}",
+           "molecules": len(inst), "basis_functions": int(sum(b.n for b in bases))}
+    path = os.path.join(tune.DATA, "batch_c4.json")
+    if os.path.exists(path):
+        t = tune.load(path)
+        batch.set_tuning(t["rows"], t["split"], t["grid"])
+        out["tuning"] = os.path.relpath(path, ROOT)
+    # the plain FP64 path, one molecule at a time
+    ref = []
+    for e, (PA, PB), x in zip(engines, dens, inst):
+        ref.append(e.build(PA, PB, cut, cut, x["restricted"]))
+    t0 = time.perf_counter()
+    for e, (PA, PB), x in zip(engines, dens, inst):
+        e.build(PA, PB, cut, cut, x["restricted"])
+    t_seq = time.perf_counter() - t0
+    out["per_molecule_fp64"] = {"seconds": t_seq, "molecule_builds_per_s": len(inst) / t_seq}
+    for label, thr in (("batched_fp64", 0.0), ("batched_mixed", mixed_thr)):
+        if label == "batched_mixed" and thr <= 0:
+            continue
+        batch.set_mixed(thr)
+        batch.build(graph=True)
+        batch.build(graph=True)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.build(graph=True)
+            kms.append(batch.stats()["kernel_ms"])
+        dt = (time.perf_counter() - t0) / steps
+        d = 0.0
+        for k in range(0, len(inst), 17):
+            J, KA, KB = batch.result(k)
+            d = max(d, float(np.abs(J - ref[k][0]).max()), float(np.abs(KA - ref[k][1]).max()),
+                    float(np.abs(KB - ref[k][2]).max()))
+        out[label] = {"ms_per_batch": dt * 1e3, "kernel_ms": float(np.mean(kms)),
+                      "molecule_builds_per_s": len(inst) / dt, "speedup_vs_per_molecule_fp64": t_seq / dt,
+                      "max_abs_diff_vs_per_molecule_fp64": d, "threshold": thr,
+                      "launches": batch.stats()["launches"]}
+    batch.set_mixed(0.0)
+    batch.close()
+    for e in engines:
+        e.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +342,8 @@ def main():
         e32.close()
         out["w32"] = {"workload": f"{desc32} RHF/6-31G*, {bas32.n} functions, screening 1e-12, seeded density",
                       "value": a.steps / w32, "unit": "J/K builds/s", "kernel_ms": k32}
+        print("bench: batched builds", file=sys.stderr, flush=True)
+        out["batch"] = batch_bench(a.steps, a.mixed_threshold)
         if a.scf_solves > 0:
             print("bench: SCF solves", file=sys.stderr, flush=True)
             out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)

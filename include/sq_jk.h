@@ -118,6 +118,48 @@ int sqjk_class_stats(const sqjk_ctx *ctx, int *lq, long long *quartets, long lon
 int sqjk_class_work(sqjk_ctx *ctx, long long *quartets, long long *prim_quartets,
                     long long *quartets_fp32, long long *prim_quartets_fp32);
 
+/* Work-plan tuning per class pair (arrays of 6 x 6 ints indexed
+ * [bra pair class][ket pair class], pair classes ss, ps, pp, ds, dp, dd; any
+ * may be NULL to keep the current values; all NULL restores the defaults):
+ *   rows          bra rows per tile (1..64),
+ *   split_budget  integral-primitive products per lane and tile above which
+ *                 a segment's ket primitive pairs are split over tiles,
+ *   grid_over     workgroups per launch as a multiple of those the device
+ *                 holds at once (0: the kernel's default).
+ * Results do not depend on them; the autotuner (sqamd/tune.py) picks them.
+ * No counterpart in the reference. */
+int sqjk_set_tuning(sqjk_ctx *ctx, const int *rows, const int *split_budget, const int *grid_over);
+int sqjk_get_tuning(const sqjk_ctx *ctx, int *rows, int *split_budget, int *grid_over);
+
+/* ---- batches: many molecules' J/K builds as one set of launches ----
+ * A batch joins existing contexts (which must outlive it; their geometry
+ * must not change while it exists).  Per (class pair, precision, spin mode)
+ * one launch covers the quartets of every molecule; the whole build is
+ * captured into a HIP graph on its first graph build and replayed while
+ * the cut-offs stay the same.  Densities and results live in the batch's
+ * device buffers.  restricted[i] selects RHF (K_B := K_A) or UHF per
+ * molecule.  No counterpart in the reference, whose J/K state is static. */
+typedef struct sqjk_batch sqjk_batch;
+int sqjk_batch_create(int count, sqjk_ctx *const *ctxs, const int *restricted, sqjk_batch **out);
+/* density of molecule i (n_i x n_i; host or device memory) */
+int sqjk_batch_set_density(sqjk_batch *b, int i, const double *PA, const double *PB, int on_device);
+/* cut-offs of molecule i (i = -1: every molecule), as in sqjk_build */
+int sqjk_batch_set_cutoffs(sqjk_batch *b, int i, double fixedCutoff, double scfCutoff);
+int sqjk_batch_set_mixed(sqjk_batch *b, double threshold);
+int sqjk_batch_set_tuning(sqjk_batch *b, const int *rows, const int *split_budget, const int *grid_over);
+/* per-class event timing of eager builds */
+int sqjk_batch_set_profile(sqjk_batch *b, int on);
+/* One build of every molecule: J = J[PA+PB], KA = -K[PA], KB = -K[PB]
+ * (full symmetric), overwritten.  use_graph: replay the captured graph
+ * (captured after an eager build on the first call of a plan). */
+int sqjk_batch_build(sqjk_batch *b, int use_graph);
+int sqjk_batch_result(sqjk_batch *b, int i, double *J, double *KA, double *KB, int on_device);
+/* device time of the last build, its screened quartets and class launches,
+ * and (profile mode, eager) per-class ms in sqjk_class_stats order */
+int sqjk_batch_stats(const sqjk_batch *b, double *kernel_ms, long long *quartets, int *launches,
+                     double *class_ms);
+void sqjk_batch_destroy(sqjk_batch *b);
+
 /* Per-class kernel timing.  With `on` set, every later sqjk_build records a
  * pair of HIP events around each class launch group on the context's stream
  * (all ket-component slices, FP64 and FP32 launches of one class) and

@@ -69,7 +69,8 @@ long long resident_blocks() {
   return v;
 }
 
-// grid <= 0: size the grid from the work, a.total = tiles (RHF/UHF) or pairs
+// grid > 0: that many workgroups; grid <= 0: size the grid from the work
+// (with -grid as the multiple of resident workgroups when grid < 0), a.total = tiles (RHF/UHF) or pairs
 // (Schwarz).  Per-thread kernel: one wavefront per tile (one lane per pair)
 // and slice, at most a few times the workgroups the device holds at once
 // over all slices, so each workgroup fills its LDS Boys table once and then
@@ -81,13 +82,14 @@ template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
     const long long items = MODE == kSchwarz ? a.total : a.total * 64;
-    const long long g = grid > 0 ? grid : (items < (1 << 20) ? items : (1 << 20));
+    const long long cap = grid < 0 ? (long long)(-grid) << 18 : (1 << 20);
+    const long long g = grid > 0 ? grid : (items < cap ? items : cap);
     return launch_wave<MODE, R>(a, (int)(g > 0 ? g : 1), st);
   } else {
     constexpr int bs = class_block<LA, LB, LC, LD>();
     const long long lanes = MODE == kSchwarz ? a.total : a.total * 64;
     const long long b = (lanes + bs - 1) / bs;
-    constexpr long long over = bs >= 512 ? 4 : 2;
+    const long long over = grid < 0 ? -grid : (bs >= 512 ? 4 : 2);
     const long long bmax = std::max(1LL, over * resident_blocks<MODE, R>() / SI::NSLICE);
     const long long g = grid > 0 ? grid : (b < bmax ? b : bmax);
     return launch_thread<MODE, R>(a, (int)(g > 0 ? g : 1), st);

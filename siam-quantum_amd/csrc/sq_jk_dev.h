@@ -32,9 +32,9 @@ struct SubShell {
 // primitive-major, so consecutive pairs of a group read consecutive records.
 struct PairRec {
   int a, b;               // sub-shell indices
-  int pa, pb;             // parent reference shells of a, b
+  int pa, pb;             // parent reference shells of a, b (of their molecule)
   int pp_off, pp_stride;  // primitive pairs
-  int npp, pad;
+  int npp, inst;          // inst: molecule of the pair in a batch (0 otherwise)
   double sch;             // Schwarz bound of the parent pair (block maximum)
 };
 
@@ -270,8 +270,8 @@ struct RunAcc {
 // see every lane.
 // RUN: J_cd and the a-side exchange go to `run` instead of memory.
 template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE, typename R = double, bool RUN = false>
-__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int ik, int nppb, int v0, int v1,
-                                                 bool valid, const double* lds,
+__host__ __device__ __forceinline__ void quartet(const ClassArgs& A, const View& V, int ib, int ik, int nppb, int v0,
+                                                 int v1, bool valid, const double* lds,
                                                  RunAcc<LA, LC, LD, CD0, CD1>* run = nullptr) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int LAB = LA + LB, L = LAB + LC + LD;
@@ -367,8 +367,9 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     }
   }
 
-  const int n = A.n;
+  const int n = V.n;
   const int fa = sA.bf_off, fb = sB.bf_off, fc = sC.bf_off, fd = sD.bf_off;
+  const int ga = sA.gbf, gb = sB.gbf, gc = sC.gbf, gd = sD.gbf;  // component scale indices
 
   if constexpr (MODE == kSchwarz) {
     // diagonal (ab|ab) of the quartet (i,i): sqrt|.| into both (a,b), (b,a)
@@ -376,7 +377,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       constexpr int cs = decltype(csI)::value;
       constexpr int ab = CD0 + cs;
       constexpr int ia = ab / NB, jb = ab % NB;
-      const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
+      const double sc = A.bscale[ga + ia] * A.bscale[gb + jb];
       const double v = sqrt(fabs((double)acc[ab][cs] * sc * sc));
       A.sch_out[(fa + ia) * n + fb + jb] = v;
       A.sch_out[(fb + jb) * n + fa + ia] = v;
@@ -389,15 +390,15 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     if (A.same && ib == ik) deg *= 0.5;
 
     double sc_a[NA], sc_b[NB];
-    sfor<NA>([&](auto I) { sc_a[I.value] = A.bscale[fa + I.value]; });
-    sfor<NB>([&](auto I) { sc_b[I.value] = A.bscale[fb + I.value]; });
+    sfor<NA>([&](auto I) { sc_a[I.value] = A.bscale[ga + I.value]; });
+    sfor<NB>([&](auto I) { sc_b[I.value] = A.bscale[gb + I.value]; });
 
     // scaled integrals, FP64 from here on
     double vv_[NAB][NS];
     sfor<NS>([&](auto csI) {
       constexpr int cs = decltype(csI)::value;
       constexpr int cd = CD0 + cs;
-      const double scd = deg * A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
+      const double scd = deg * A.bscale[gc + cd / ND] * A.bscale[gd + cd % ND];
       sfor<NAB>([&](auto abI) {
         constexpr int ab = decltype(abI)::value;
         vv_[ab][cs] = (double)acc[ab][cs] * scd * sc_a[ab / NB] * sc_b[ab % NB];
@@ -408,12 +409,12 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     double dcd[NS], jcd[NS], jab[NAB];
     sfor<NS>([&](auto csI) {
       constexpr int cd = CD0 + decltype(csI)::value;
-      dcd[csI.value] = A.DJ[(fc + cd / ND) * n + fd + cd % ND];
+      dcd[csI.value] = V.DJ[(fc + cd / ND) * n + fd + cd % ND];
       jcd[csI.value] = 0.0;
     });
     sfor<NAB>([&](auto abI) {
       constexpr int ab = decltype(abI)::value;
-      const double dab = A.DJ[(fa + ab / NB) * n + fb + ab % NB];
+      const double dab = V.DJ[(fa + ab / NB) * n + fb + ab % NB];
       double s = 0.0;
       sfor<NS>([&](auto csI) {
         constexpr int cs = decltype(csI)::value;
@@ -445,7 +446,7 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     } else if (valid) {
       sfor<NS>([&](auto csI) {
         constexpr int cd = CD0 + decltype(csI)::value;
-        fatomic(&A.JpK[(size_t)cd * A.npk + ik], 2.0 * jcd[csI.value]);
+        fatomic(&A.JpK[(size_t)cd * A.npk + ik], 2.0 * jcd[csI.value]);  // class-list index
       });
     }
 
@@ -453,8 +454,10 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
     //                      A_K[a,d] += v D[b,c]; A_K[b,d] += v D[a,c]
     // staged as S[row][comp][ket pair] with comp = c component, or NC + d
     // component; a later pass adds S into the rows of A_K.
+    // exchange staging of the molecule: its ket pairs [k0, k0 + npk)
     constexpr int NCOMP = NC + ND;
-    const size_t npk = (size_t)A.npk;
+    const size_t npk = (size_t)V.npk;
+    const int jk = ik - V.k0;
     auto exchange = [&](const double* __restrict__ D, double* S, auto spinI) {
       using KR = KetRange<ND, CD0, CD1>;
       constexpr int C0 = KR::C0, NCr = KR::NCr, D0 = KR::D0, NDr = KR::NDr;
@@ -490,41 +493,42 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, int ib, int
       if constexpr (!RUN) {
         sfor<NA>([&](auto I) {
           const size_t row = (size_t)(fa + I.value) * NCOMP;
-          sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kac[I.value][J.value]); });
-          sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kad[I.value][J.value]); });
+          sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + jk], kac[I.value][J.value]); });
+          sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + jk], kad[I.value][J.value]); });
         });
       }
       sfor<NB>([&](auto I) {
         const size_t row = (size_t)(fb + I.value) * NCOMP;
-        sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + ik], kbc[I.value][J.value]); });
-        sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + ik], kbd[I.value][J.value]); });
+        sfor<NCr>([&](auto J) { fatomic(&S[(row + C0 + J.value) * npk + jk], kbc[I.value][J.value]); });
+        sfor<NDr>([&](auto J) { fatomic(&S[(row + NC + D0 + J.value) * npk + jk], kbd[I.value][J.value]); });
       });
     };
-    exchange(A.DA, A.SA, ic<0>{});
-    if constexpr (MODE == kUHF) exchange(A.DB, A.SB, ic<1>{});
+    exchange(V.DA, V.SA, ic<0>{});
+    if constexpr (MODE == kUHF) exchange(V.DB, V.SB, ic<1>{});
   }
 }
 
 // Writes a run's register contributions (ket pair ik, bra shell starting at
 // basis function fa) to the staging buffers.
 template <int LA, int LB, int LC, int LD, int CD0, int CD1, int MODE>
-__host__ __device__ __forceinline__ void run_flush(const ClassArgs& A, const RunAcc<LA, LC, LD, CD0, CD1>& run,
-                                                   int fa, int ik) {
+__host__ __device__ __forceinline__ void run_flush(const ClassArgs& A, const View& V,
+                                                   const RunAcc<LA, LC, LD, CD0, CD1>& run, int fa, int ik) {
   constexpr int NA = ncart(LA), NC = ncart(LC), ND = ncart(LD), NCOMP = NC + ND;
   using KR = KetRange<ND, CD0, CD1>;
-  const size_t npk = (size_t)A.npk;
+  const size_t npk = (size_t)V.npk;
+  const int jk = ik - V.k0;
   sfor<CD1 - CD0>([&](auto csI) {
     constexpr int cd = CD0 + decltype(csI)::value;
-    fatomic(&A.JpK[(size_t)cd * npk + ik], 2.0 * run.jcd[csI.value]);
+    fatomic(&A.JpK[(size_t)cd * A.npk + ik], 2.0 * run.jcd[csI.value]);
   });
   sfor<(MODE == kUHF) ? 2 : 1>([&](auto spI) {
     constexpr int sp = decltype(spI)::value;
-    double* S = sp ? A.SB : A.SA;
+    double* S = sp ? V.SB : V.SA;
     sfor<NA>([&](auto I) {
       const size_t row = (size_t)(fa + I.value) * NCOMP;
-      sfor<KR::NCr>([&](auto J) { fatomic(&S[(row + KR::C0 + J.value) * npk + ik], run.kac[sp][I.value][J.value]); });
+      sfor<KR::NCr>([&](auto J) { fatomic(&S[(row + KR::C0 + J.value) * npk + jk], run.kac[sp][I.value][J.value]); });
       sfor<KR::NDr>([&](auto J) {
-        fatomic(&S[(row + NC + KR::D0 + J.value) * npk + ik], run.kad[sp][I.value][J.value]);
+        fatomic(&S[(row + NC + KR::D0 + J.value) * npk + jk], run.kad[sp][I.value][J.value]);
       });
     });
   });
@@ -532,13 +536,12 @@ __host__ __device__ __forceinline__ void run_flush(const ClassArgs& A, const Run
 
 // Density-weighted screening at the reference-shell level (the reference's
 // six-block sum and test, src/matrix.c:3921-3924, 3936, 3943).
-__host__ __device__ __forceinline__ bool density_screen_pass(const ClassArgs& A, const PairRec& bp,
-                                                             const PairRec& kp) {
-  const double* PM = A.PM;
-  const int nr = A.nref;
+__host__ __device__ __forceinline__ bool density_screen_pass(const View& V, const PairRec& bp, const PairRec& kp) {
+  const double* PM = V.PM;
+  const int nr = V.nref;
   const double mp = uload(PM + (bp.pa * nr + bp.pb)) + PM[kp.pa * nr + kp.pb] + PM[bp.pa * nr + kp.pa] +
                     PM[bp.pa * nr + kp.pb] + PM[bp.pb * nr + kp.pa] + PM[bp.pb * nr + kp.pb];
-  return !(bp.sch * kp.sch * mp < A.fixed_cut);
+  return !(bp.sch * kp.sch * mp < V.fixed_cut);
 }
 
 // One launch per (class pair, precision, mode); blockIdx.y selects the
@@ -567,7 +570,8 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
       const long long stride = (long long)gridDim.x * blockDim.x;
       for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += stride) {
         const int npp = A.bra[t].npp;
-        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, (int)t, (int)t, npp, 0, npp, true, lds);
+        const View V = make_view(A, 0);
+        quartet<LA, LB, LC, LD, CD0, CD1, MODE, R>(A, V, (int)t, (int)t, npp, 0, npp, true, lds);
       }
     } else {
       const int lane = threadIdx.x & 63;
@@ -577,6 +581,7 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
       for (long long w = (long long)blockIdx.x * wpb + wib; w < A.total; w += wstride) {
         const Tile tl = uload(A.tile + w);
         const Segment sg = uload(A.seg + tl.seg);
+        const View V = make_view(A, sg.inst);
         const int j = tl.k0 + lane;  // ket position in the group
         RunAcc<LA, LC, LD, CD0, CD1> run;
         run.zero();
@@ -587,11 +592,13 @@ __global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
           const bool in = j >= lo && j < uload(A.rhi + r);
           const int ik = sg.ket0 + (in ? j : lo);  // idle lanes evaluate a valid pair, weight 0
           any = any || in;
-          const bool valid = in && density_screen_pass(A, uload(A.bra + ib), A.ket[ik]);
+          const bool valid = in && density_screen_pass(V, uload(A.bra + ib), A.ket[ik]);
           if (!__any(valid)) continue;  // the whole wavefront screened out: skip the row (uniform)
-          quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, ib, ik, sg.nppb, tl.v0, tl.v1, valid, lds, &run);
+          quartet<LA, LB, LC, LD, CD0, CD1, MODE, R, true>(A, V, ib, ik, sg.nppb, tl.v0, tl.v1, valid, lds, &run);
         }
-        if (any) run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, run, uload(A.sh + (uload(A.bra + (uload(A.rbra + tl.r0))).a)).bf_off, sg.ket0 + j);
+        if (any)
+          run_flush<LA, LB, LC, LD, CD0, CD1, MODE>(A, V, run, uload(A.sh + (uload(A.bra + (uload(A.rbra + tl.r0))).a)).bf_off,
+                                                    sg.ket0 + j);
       }
     }
   });

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -225,12 +226,13 @@ __global__ void __launch_bounds__(256) k_count_work(ClassArgs A, unsigned long l
   for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < A.total; w += wstride) {
     const Tile tl = A.tile[w];
     const Segment sg = A.seg[tl.seg];
+    const View V = make_view(A, sg.inst);
     const int j = tl.k0 + lane;
     for (int r = tl.r0; r < tl.r1; ++r) {
       if (j < A.rlo[r] || j >= A.rhi[r]) continue;
       const int ib = A.rbra[r], ik = sg.ket0 + j;
       const PairRec bp = A.bra[ib], kp = A.ket[ik];
-      if (!density_screen_pass(A, bp, kp)) continue;
+      if (!density_screen_pass(V, bp, kp)) continue;
       if (tl.v0 == 0) q += 1.0;
       for (int u = 0; u < sg.nppb; ++u) {
         const PrimPair P = A.pp[bp.pp_off + u * bp.pp_stride];
@@ -332,6 +334,12 @@ struct sqjk_ctx {
   // per class pair (X,Y) of the last build: contracted and primitive quartets
   long long cls_q[kNPairClass][kNPairClass] = {};
   long long cls_p[kNPairClass][kNPairClass] = {};
+  // work-plan tuning per class pair: bra rows per tile, integral-primitive
+  // products per lane and tile before a segment is split, and the grid's
+  // multiple of the resident workgroups (0: the kernel's default)
+  int rows[kNPairClass][kNPairClass] = {};
+  int split_budget[kNPairClass][kNPairClass] = {};
+  int grid_over[kNPairClass][kNPairClass] = {};
   double last_fixed = 0.0;  // density-weighted cut-off of the last build
   bool timing_pending = false;  // the last build's events are not yet read
   bool built = false;       // a build has run (PM holds its density maxima)
@@ -448,7 +456,7 @@ int plan_shells(sqjk_ctx* c, int n, const GTOBasis_t* g, std::vector<double>& pe
       }
       SubShell s{};
       s.x = g[f].x0; s.y = g[f].y0; s.z = g[f].z0;
-      s.l = l; s.nprim = g[f].nContract; s.prim_off = (int)pexp.size(); s.bf_off = f; s.parent = r;
+      s.l = l; s.nprim = g[f].nContract; s.prim_off = (int)pexp.size(); s.bf_off = f; s.parent = r; s.gbf = f;
       // contraction factorises as coef*norm = ctilde[k] * scale[component]
       for (int k = 0; k < s.nprim; ++k) {
         const double c0 = g[f].coef[k] * g[f].norm[k];
@@ -669,9 +677,10 @@ void host_parallel(int maxt, F&& fn) {
 // six-primitive core shells (1296 primitive quartets per lane) or the (pp|pp)
 // blocks of contracted p shells otherwise set the length of a whole launch
 // for small molecules.
-int split_prims(int nint) {
-  return std::max(nint > 81 ? 32 : 8, std::min(64, 576 / std::max(1, nint)));
+int split_prims(int nint, int budget) {
+  return std::max(nint > 81 ? 32 : 8, std::min(64, budget / std::max(1, nint)));
 }
+constexpr int kSplitBudget = 576;
 
 // Bra rows per tile for the class pair (X, Y) (pair classes ss, ps, pp, ds,
 // dp, dd).  Two rows keep J_cd and the a-side exchange of the lane's ket pair
@@ -682,7 +691,7 @@ int split_prims(int nint) {
 // Measured per class at 1, 2 and 3 rows on MI355X, (H2O)32 6-31G*: 16.7 ms
 // per build with one row everywhere, 14.8 ms with two rows everywhere.
 // SQJK_TILE_ROWS overrides the table.
-int tile_rows(int X, int Y) {
+int default_tile_rows(int X, int Y) {
   static const int env = [] {
     const char* e = std::getenv("SQJK_TILE_ROWS");
     const int r = e ? std::atoi(e) : 0;
@@ -698,6 +707,15 @@ int tile_rows(int X, int Y) {
       {2, 2, 1, 2, 2, 0},   // X = dp
       {1, 1, 1, 1, 1, 1}};  // X = dd
   return kRows[X][Y] > 0 ? kRows[X][Y] : 1;
+}
+
+void init_tuning(sqjk_ctx* c) {
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y < kNPairClass; ++Y) {
+      c->rows[X][Y] = default_tile_rows(X, Y);
+      c->split_budget[X][Y] = kSplitBudget;
+      c->grid_over[X][Y] = 0;
+    }
 }
 
 void build_plan(sqjk_ctx* c, double thr, double thr64) {
@@ -779,7 +797,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                 sel.clear();
                 for (size_t q = r; q < e; ++q)
                   if (rows[q].lo < k0 + 64 && rows[q].hi > k0) sel.push_back(rows[q]);
-                const size_t tr = (size_t)tile_rows(X, Y);
+                const size_t tr = (size_t)c->rows[X][Y];
                 for (size_t q = 0; q < sel.size(); q += tr) {
                   const int r0 = (int)c->h_rbra.size();
                   for (size_t u = q; u < std::min(sel.size(), q + tr); ++u) {
@@ -789,7 +807,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
                   }
                   // a segment of more than kSplitPrims primitive quartets per
                   // quartet is cut along its ket primitive pairs
-                  const int sp = split_prims(nint);
+                  const int sp = split_prims(nint, c->split_budget[X][Y]);
                   const int nch = std::min(g2.npp, (g1.npp * g2.npp + sp - 1) / sp);
                   for (int ch = 0; ch < nch; ++ch)
                     c->h_tile.push_back({seg, r0, (int)c->h_rbra.size(), k0, g2.npp * ch / nch,
@@ -865,6 +883,7 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   sqjk_ctx* c = new sqjk_ctx();
   c->device = device;
   c->n = nBasis;
+  init_tuning(c);
   std::vector<double> pexp, pcoef;
   int rc = plan_shells(c, nBasis, gto, pexp, pcoef);
   if (rc) { delete c; return rc; }
@@ -1064,7 +1083,7 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
     a.npb = (int)c->pairs[L.X].size();
     a.npk = (int)c->pairs[L.Y].size();
     ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
-    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, 0, c->st) : -1;
+    const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, -c->grid_over[L.X][L.Y], c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
     if (c->profile) HIPCHK(hipEventRecord(c->prof_ev[2 * li + 1], c->st));
@@ -1323,6 +1342,40 @@ int sqjk_class_work(sqjk_ctx* c, long long* quartets, long long* prim_quartets, 
   return k;
 }
 
+int sqjk_set_tuning(sqjk_ctx* c, const int* rows, const int* split_budget, const int* grid_over) {
+  if (!c) return fail(SQJK_EARG, "sqjk_set_tuning: null context");
+  for (int k = 0; k < kNPairClass * kNPairClass; ++k) {
+    if (rows && (rows[k] < 1 || rows[k] > 64)) return fail(SQJK_EARG, "sqjk_set_tuning: rows per tile outside [1, 64]");
+    if (split_budget && (split_budget[k] < 1 || split_budget[k] > 1 << 20))
+      return fail(SQJK_EARG, "sqjk_set_tuning: split budget outside [1, 2^20]");
+    if (grid_over && (grid_over[k] < 0 || grid_over[k] > 64))
+      return fail(SQJK_EARG, "sqjk_set_tuning: grid multiple outside [0, 64]");
+  }
+  if (!rows && !split_budget && !grid_over) {
+    init_tuning(c);
+  } else {
+    for (int X = 0; X < kNPairClass; ++X)
+      for (int Y = 0; Y < kNPairClass; ++Y) {
+        if (rows) c->rows[X][Y] = rows[X * kNPairClass + Y];
+        if (split_budget) c->split_budget[X][Y] = split_budget[X * kNPairClass + Y];
+        if (grid_over) c->grid_over[X][Y] = grid_over[X * kNPairClass + Y];
+      }
+  }
+  c->plan_ok = false;
+  return SQJK_OK;
+}
+
+int sqjk_get_tuning(const sqjk_ctx* c, int* rows, int* split_budget, int* grid_over) {
+  if (!c) return fail(SQJK_EARG, "sqjk_get_tuning: null context");
+  for (int X = 0; X < kNPairClass; ++X)
+    for (int Y = 0; Y < kNPairClass; ++Y) {
+      if (rows) rows[X * kNPairClass + Y] = c->rows[X][Y];
+      if (split_budget) split_budget[X * kNPairClass + Y] = c->split_budget[X][Y];
+      if (grid_over) grid_over[X * kNPairClass + Y] = c->grid_over[X][Y];
+    }
+  return SQJK_OK;
+}
+
 int sqjk_set_profile(sqjk_ctx* c, int on) {
   if (!c) return fail(SQJK_EARG, "sqjk_set_profile: null context");
   c->profile = on != 0;
@@ -1370,6 +1423,631 @@ void sqjk_destroy(sqjk_ctx* c) {
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
+
+// ------------------------------------------------------------ batches
+// Many molecules' J/K builds as one set of launches: one launch per (class
+// pair, precision, spin mode) covers the tiles of every molecule of the
+// batch.  The molecules' pair lists, primitive data and component scales
+// are concatenated (pair records carry their molecule, sub-shells their
+// offset in the concatenated scales), the tiles of each molecule keep their
+// own work plan, and a segment names its molecule, whose matrices,
+// screening maxima and exchange staging the kernels find through a
+// per-molecule descriptor.  The whole build (density preparation, fills,
+// class launches, reductions, finalize) is captured once into a HIP graph
+// and replayed while the work plan stays the same.
+
+namespace {
+
+// Per-molecule offsets of the concatenated helper arrays
+struct BatchGeom {
+  long long mat_off;  // n x n blocks
+  long long pm_off;   // nref x nref blocks
+  int ref0;           // first reference shell in the concatenated list
+  int row0;           // first basis function (rows of all molecules)
+  int n, nref, uhf;
+};
+
+__host__ __device__ inline int find_inst(const long long* off, int count, long long e) {  // last i with off[i] <= e
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_pair_max_batch(const double* __restrict__ DJ, const double* __restrict__ DA,
+                                 const double* __restrict__ DB, const int* __restrict__ ref,
+                                 const BatchGeom* __restrict__ geo, const long long* __restrict__ pm_off, int count,
+                                 long long total, double* PM) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = find_inst(pm_off, count, t);
+    const BatchGeom g = geo[i];
+    const long long l = t - g.pm_off;
+    const int P = (int)(l / g.nref), Q = (int)(l - (long long)P * g.nref);
+    const int* rf = ref + 2 * g.ref0;
+    const int p0 = rf[2 * P], np = rf[2 * P + 1], q0 = rf[2 * Q], nq = rf[2 * Q + 1];
+    const double* dj = DJ + g.mat_off;
+    const double* da = DA + g.mat_off;
+    const double* db = DB + g.mat_off;
+    double r = 0.0;
+    for (int p = p0; p < p0 + np; ++p)
+      for (int q = q0; q < q0 + nq; ++q) {
+        const size_t e = (size_t)p * g.n + q;
+        r = fmax(r, fmax(fabs(dj[e]), fmax(fabs(da[e]), fabs(db[e]))));
+      }
+    PM[t] = r;
+  }
+}
+
+// A_J of each pair's molecule += Jp (see k_reduce_j)
+__global__ void k_reduce_j_batch(const double* __restrict__ Jp, const PairRec* __restrict__ pr,
+                                 const SubShell* __restrict__ sh, const InstDesc* __restrict__ inst, int np, int nb,
+                                 int nab) {
+  const long long tot = (long long)np * nab;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double v = Jp[e];
+    if (v == 0.0) continue;
+    const int ab = (int)(e / np), i = (int)(e - (long long)ab * np);
+    const PairRec p = pr[i];
+    const InstDesc& d = inst[p.inst];
+    d.AJ[(size_t)(sh[p.a].bf_off + ab / nb) * d.n + sh[p.b].bf_off + ab % nb] += v;
+  }
+}
+
+// One workgroup per row of every molecule's A_K (see k_reduce_k); spin 1
+// rows of RHF molecules are skipped.
+__global__ void __launch_bounds__(kReduceKBlock) k_reduce_k_batch(const PairRec* const* __restrict__ prs,
+                                                                  const int* __restrict__ ncomp_c,
+                                                                  const int* __restrict__ nc_c,
+                                                                  const SubShell* __restrict__ sh,
+                                                                  const InstDesc* __restrict__ inst,
+                                                                  const BatchGeom* __restrict__ geo,
+                                                                  const long long* __restrict__ row0, int count,
+                                                                  int rows, int spin) {
+  extern __shared__ double row[];
+  for (int xg = blockIdx.x; xg < rows; xg += gridDim.x) {
+    const int i = find_inst(row0, count, xg);
+    const BatchGeom g = geo[i];
+    if (spin == 1 && !g.uhf) continue;  // uniform in the workgroup
+    const int x = xg - g.row0, n = g.n;
+    const InstDesc& d = inst[i];
+    for (int y = threadIdx.x; y < n; y += blockDim.x) row[y] = 0.0;
+    __syncthreads();
+    for (int Y = 0; Y < 6; ++Y) {
+      const double* S = spin ? d.SB[Y] : d.SA[Y];
+      const int np = d.np[Y];
+      if (!S || np == 0) continue;
+      const int nc = nc_c[Y], ncomp = ncomp_c[Y];
+      const double* Sx = S + (size_t)x * ncomp * np;
+      const PairRec* prY = prs[Y] + d.p0[Y];
+      const long long tot = (long long)ncomp * np;
+      for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+        const double v = Sx[e];
+        if (v == 0.0) continue;
+        const int comp = (int)(e / np), j = (int)(e - (long long)comp * np);
+        const PairRec p = prY[j];
+        const int col = comp < nc ? sh[p.a].bf_off + comp : sh[p.b].bf_off + comp - nc;
+        __hip_atomic_fetch_add(&row[col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    double* AK = spin ? d.AKB : d.AKA;
+    for (int y = threadIdx.x; y < n; y += blockDim.x) AK[(size_t)x * n + y] += row[y];
+    __syncthreads();
+  }
+}
+
+// J = A + A^T, K = -(A + A^T) per molecule; KB := KA for RHF molecules
+__global__ void k_finalize_batch(const double* __restrict__ AJ, const double* __restrict__ AKA,
+                                 const double* __restrict__ AKB, double* J, double* KA, double* KB,
+                                 const BatchGeom* __restrict__ geo, const long long* __restrict__ mat_off, int count,
+                                 long long total) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = find_inst(mat_off, count, e);
+    const BatchGeom g = geo[i];
+    const long long l = e - g.mat_off;
+    const int r = (int)(l / g.n), c = (int)(l - (long long)r * g.n);
+    const long long t = g.mat_off + (long long)c * g.n + r;
+    J[e] = AJ[e] + AJ[t];
+    const double ka = -(AKA[e] + AKA[t]);
+    KA[e] = ka;
+    KB[e] = g.uhf ? -(AKB[e] + AKB[t]) : ka;
+  }
+}
+
+}  // namespace
+
+struct sqjk_batch {
+  int device = 0, count = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<sqjk_ctx*> ctx;
+  std::vector<int> uhf;
+  std::vector<double> fixed, scf;  // per-molecule cut-offs of the next build
+  double mixed_thr = 0.0;
+  // concatenated device data
+  SubShell* d_sub = nullptr;
+  PrimPair* d_pp = nullptr;
+  PairRec* d_pairs[kNPairClass] = {};
+  const PairRec** d_prs = nullptr;  // the six class lists (for the K reduction)
+  int* d_ncomp = nullptr;           // per pair class: components, c-components
+  int* d_nc = nullptr;
+  double *d_bscale = nullptr, *d_boys = nullptr;
+  int* d_ref = nullptr;
+  long long mat_total = 0, pm_total = 0;
+  int rows_total = 0;
+  double *d_DA = nullptr, *d_DB = nullptr, *d_DJ = nullptr, *d_PM = nullptr;
+  double *d_AJ = nullptr, *d_AKA = nullptr, *d_AKB = nullptr, *d_J = nullptr, *d_KA = nullptr, *d_KB = nullptr;
+  double* d_Jp[kNPairClass] = {};
+  size_t jp_len[kNPairClass] = {};
+  int np_total[kNPairClass] = {};
+  double *d_SA = nullptr, *d_SB = nullptr;
+  size_t s_total = 0;
+  std::vector<BatchGeom> geo;
+  BatchGeom* d_geo = nullptr;
+  long long *d_mat_off = nullptr, *d_pm_off = nullptr, *d_row0 = nullptr;
+  std::vector<InstDesc> desc;
+  InstDesc* d_inst = nullptr;
+  // merged work plan
+  struct Launch { int X, Y, fp32, mode; long long tile0, ntile; };
+  std::vector<Launch> launches;
+  Segment* d_seg = nullptr;
+  Tile* d_tile = nullptr;
+  int *d_rbra = nullptr, *d_rlo = nullptr, *d_rhi = nullptr;
+  bool plan_ok = false;
+  long long plan_q = 0, plan_q32 = 0;
+  // graph of the build for the current plan
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool profile = false;
+  std::vector<hipEvent_t> prof_ev;
+  double cls_ms[kNPairClass][kNPairClass] = {};
+  double last_ms = 0.0;
+  int last_launches = 0;
+};
+
+namespace {
+
+void batch_free_plan(sqjk_batch* b) {
+  void* p[] = {b->d_seg, b->d_tile, b->d_rbra, b->d_rlo, b->d_rhi};
+  for (void* q : p) block_put(q);
+  b->d_seg = nullptr; b->d_tile = nullptr; b->d_rbra = b->d_rlo = b->d_rhi = nullptr;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->plan_ok = false;
+}
+
+// Per-molecule plans for the molecules' cut-offs, merged per (class pair,
+// precision, spin mode): segments get their molecule and the class-list
+// offset of its ket pairs, rows the offset of its bra pairs, tiles the
+// offsets of the merged segments and rows.
+int batch_plan(sqjk_batch* b) {
+  batch_free_plan(b);
+  std::vector<Segment> seg;
+  std::vector<Tile> tile;
+  std::vector<int> rbra, rlo, rhi;
+  b->launches.clear();
+  b->plan_q = b->plan_q32 = 0;
+  std::vector<std::array<int, kNPairClass>> p0(b->count);
+  {
+    std::array<int, kNPairClass> acc{};
+    for (int i = 0; i < b->count; ++i) {
+      p0[i] = acc;
+      for (int X = 0; X < kNPairClass; ++X) acc[X] += (int)b->ctx[i]->pairs[X].size();
+    }
+  }
+  for (int i = 0; i < b->count; ++i) {
+    sqjk_ctx* c = b->ctx[i];
+    c->mixed_thr = b->mixed_thr;
+    const double thr = std::max(b->fixed[i], b->scf[i]);
+    build_plan(c, thr, std::max(thr, c->mixed_thr));
+    c->plan_ok = false;  // the context's own device plan is not this one
+  }
+  for (int mode = 0; mode < 2; ++mode)
+    for (int X = 0; X < kNPairClass; ++X)
+      for (int Y = 0; Y <= X; ++Y)
+        for (int fp = 0; fp < 2; ++fp) {
+          sqjk_batch::Launch L{X, Y, fp, mode, (long long)tile.size(), 0};
+          for (int i = 0; i < b->count; ++i) {
+            if (b->uhf[i] != mode) continue;
+            const sqjk_ctx* c = b->ctx[i];
+            for (const sqjk_ctx::Launch& l : c->launches) {
+              if (l.X != X || l.Y != Y || l.fp32 != fp) continue;
+              const int seg_base = (int)seg.size() - l.seg0, row_base = (int)rbra.size();
+              const int seg_end = (&l == &c->launches.back()) ? (int)c->h_seg.size() : (&l + 1)->seg0;
+              for (int sgi = l.seg0; sgi < seg_end; ++sgi) {
+                Segment sgm = c->h_seg[sgi];
+                sgm.ket0 += p0[i][Y];
+                sgm.inst = i;
+                seg.push_back(sgm);
+              }
+              const long long t_end = l.tile0 + l.ntile;
+              const int r_first = c->h_tile[l.tile0].r0;
+              const int r_last = c->h_tile[t_end - 1].r1;
+              for (int r = r_first; r < r_last; ++r) {
+                rbra.push_back(c->h_rbra[r] + p0[i][X]);
+                rlo.push_back(c->h_rlo[r]);
+                rhi.push_back(c->h_rhi[r]);
+              }
+              for (long long t = l.tile0; t < t_end; ++t) {
+                Tile tl = c->h_tile[t];
+                tl.seg += seg_base;
+                tl.r0 += row_base - r_first;
+                tl.r1 += row_base - r_first;
+                tile.push_back(tl);
+              }
+              b->plan_q += l.quartets;
+              if (fp) b->plan_q32 += l.quartets;
+            }
+          }
+          L.ntile = (long long)tile.size() - L.tile0;
+          if (L.ntile > 0) b->launches.push_back(L);
+        }
+  if (rbra.size() > (size_t)INT32_MAX || seg.size() > (size_t)INT32_MAX)
+    return fail(SQJK_ENOMEM, "batched work plan too large");
+  int rc;
+  if ((rc = dalloc(&b->d_seg, seg.size())) || (rc = dalloc(&b->d_tile, tile.size())) ||
+      (rc = dalloc(&b->d_rbra, rbra.size())) || (rc = dalloc(&b->d_rlo, rlo.size())) ||
+      (rc = dalloc(&b->d_rhi, rhi.size())))
+    return rc;
+  if (!tile.empty()) {
+    HIPCHK(hipMemcpyAsync(b->d_seg, seg.data(), seg.size() * sizeof(Segment), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->d_tile, tile.data(), tile.size() * sizeof(Tile), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->d_rbra, rbra.data(), rbra.size() * sizeof(int), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->d_rlo, rlo.data(), rlo.size() * sizeof(int), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->d_rhi, rhi.data(), rhi.size() * sizeof(int), hipMemcpyHostToDevice, b->st));
+  }
+  for (int i = 0; i < b->count; ++i) b->desc[i].fixed_cut = b->fixed[i];
+  HIPCHK(hipMemcpyAsync(b->d_inst, b->desc.data(), b->desc.size() * sizeof(InstDesc), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  b->plan_ok = true;
+  return SQJK_OK;
+}
+
+// The build on the batch's stream (captured into the graph, or eager; the
+// per-class events of profile mode only when eager).
+int batch_enqueue(sqjk_batch* b, bool prof) {
+  const long long mt = b->mat_total;
+  const int fill_grid = grid_for(mt, 256);
+  hipLaunchKernelGGL(k_density_sum, dim3(fill_grid), dim3(256), 0, b->st, b->d_DA, b->d_DB, b->d_DJ, (size_t)mt);
+  hipLaunchKernelGGL(k_pair_max_batch, dim3(grid_for(b->pm_total, 256)), dim3(256), 0, b->st, b->d_DJ, b->d_DA,
+                     b->d_DB, b->d_ref, b->d_geo, b->d_pm_off, b->count, b->pm_total, b->d_PM);
+  hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, b->st, b->d_AJ, (size_t)mt, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, b->st, b->d_AKA, (size_t)mt, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, b->st, b->d_AKB, (size_t)mt, 0.0);
+  for (int X = 0; X < kNPairClass; ++X)
+    if (b->jp_len[X])
+      hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)b->jp_len[X], 256)), dim3(256), 0, b->st, b->d_Jp[X],
+                         b->jp_len[X], 0.0);
+  const int sg = std::max(1, (int)std::min<size_t>((b->s_total + 255) / 256, 16384));
+  hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, b->st, b->d_SA, b->s_total, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, b->st, b->d_SB, b->s_total, 0.0);
+  HIPCHK(hipGetLastError());
+  int launches = 0;
+  for (size_t li = 0; li < b->launches.size(); ++li) {
+    const sqjk_batch::Launch& L = b->launches[li];
+    if (prof) HIPCHK(hipEventRecord(b->prof_ev[2 * li], b->st));
+    ClassArgs a{};
+    a.sh = b->d_sub; a.pp = b->d_pp; a.boys = b->d_boys; a.bscale = b->d_bscale;
+    a.bra = b->d_pairs[L.X];
+    a.ket = b->d_pairs[L.Y];
+    a.seg = b->d_seg;
+    a.tile = b->d_tile + L.tile0;
+    a.rbra = b->d_rbra; a.rlo = b->d_rlo; a.rhi = b->d_rhi;
+    a.same = (L.X == L.Y);
+    a.total = L.ntile;
+    a.JpB = b->d_Jp[L.X];
+    a.JpK = b->d_Jp[L.Y];
+    a.npb = b->np_total[L.X];
+    a.npk = b->np_total[L.Y];
+    a.inst = b->d_inst;
+    a.ycls = L.Y;
+    const int go = b->ctx[0]->grid_over[L.X][L.Y];
+    ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
+    const int k = f ? f(a, L.mode ? kUHF : kRHF, L.fp32, -go, b->st) : -1;
+    if (k < 0) return fail(SQJK_EHIP, "batched class kernel launch failed");
+    launches += k;
+    if (prof) HIPCHK(hipEventRecord(b->prof_ev[2 * li + 1], b->st));
+  }
+  for (int X = 0; X < kNPairClass; ++X) {
+    const int np = b->np_total[X];
+    if (np == 0) continue;
+    const int nb = sqmd::ncart(kPairL[X][1]), nab = sqmd::ncart(kPairL[X][0]) * nb;
+    hipLaunchKernelGGL(k_reduce_j_batch, dim3(grid_for((long long)np * nab, 256)), dim3(256), 0, b->st, b->d_Jp[X],
+                       b->d_pairs[X], b->d_sub, b->d_inst, np, nb, nab);
+  }
+  int nmax = 0;
+  for (const BatchGeom& g : b->geo) nmax = std::max(nmax, g.n);
+  const int kgrid = std::min(b->rows_total, 65535);
+  for (int spin = 0; spin < 2; ++spin)
+    hipLaunchKernelGGL(k_reduce_k_batch, dim3(kgrid), dim3(kReduceKBlock), (size_t)nmax * sizeof(double), b->st,
+                       b->d_prs, b->d_ncomp, b->d_nc, b->d_sub, b->d_inst, b->d_geo, b->d_row0, b->count,
+                       b->rows_total, spin);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize_batch, dim3(fill_grid), dim3(256), 0, b->st, b->d_AJ, b->d_AKA, b->d_AKB, b->d_J,
+                     b->d_KA, b->d_KB, b->d_geo, b->d_mat_off, b->count, mt);
+  HIPCHK(hipGetLastError());
+  b->last_launches = launches;
+  return SQJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sqjk_batch_create(int count, sqjk_ctx* const* ctxs, const int* restricted, sqjk_batch** out) {
+  if (!out || count <= 0 || !ctxs || !restricted) return fail(SQJK_EARG, "sqjk_batch_create: invalid arguments");
+  *out = nullptr;
+  const int dev = ctxs[0]->device;
+  for (int i = 0; i < count; ++i)
+    if (!ctxs[i] || ctxs[i]->device != dev) return fail(SQJK_EARG, "sqjk_batch_create: contexts must share one device");
+  HIPCHK(hipSetDevice(dev));
+  sqjk_batch* b = new sqjk_batch();
+  b->device = dev;
+  b->count = count;
+  b->ctx.assign(ctxs, ctxs + count);
+  b->uhf.resize(count);
+  for (int i = 0; i < count; ++i) b->uhf[i] = restricted[i] ? 0 : 1;
+  b->fixed.assign(count, 0.0);
+  b->scf.assign(count, 0.0);
+  auto bail = [&](int rc) { sqjk_batch_destroy(b); return rc; };
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(SQJK_EHIP, "batch stream/event creation failed"));
+  // concatenated sub-shells, pairs, primitive data, scales, reference shells
+  std::vector<SubShell> sub;
+  std::vector<PrimPair> pp;
+  std::vector<PairRec> pairs[kNPairClass];
+  std::vector<double> bscale;
+  std::vector<int> ref;
+  std::vector<long long> mat_off, pm_off, row0;
+  b->geo.resize(count);
+  b->desc.resize(count);
+  long long mat = 0, pm = 0;
+  int rows = 0;
+  std::vector<std::array<size_t, kNPairClass>> s_off(count);
+  size_t s_acc = 0;
+  int np_acc[kNPairClass] = {};
+  for (int i = 0; i < count; ++i) {
+    const sqjk_ctx* c = ctxs[i];
+    const int sub0 = (int)sub.size(), pp0 = (int)pp.size(), bf0 = (int)bscale.size();
+    for (SubShell s : c->sub) {
+      s.gbf = s.bf_off + bf0;
+      sub.push_back(s);
+    }
+    pp.insert(pp.end(), c->pp.begin(), c->pp.end());
+    bscale.insert(bscale.end(), c->bscale.begin(), c->bscale.end());
+    BatchGeom& g = b->geo[i];
+    g.mat_off = mat; g.pm_off = pm; g.ref0 = (int)(ref.size() / 2); g.row0 = rows;
+    g.n = c->n; g.nref = c->nref; g.uhf = b->uhf[i];
+    for (const RefShell& r : c->ref) { ref.push_back(r.first); ref.push_back(r.count); }
+    mat_off.push_back(mat); pm_off.push_back(pm); row0.push_back(rows);
+    InstDesc& d = b->desc[i];
+    d.n = c->n; d.nref = c->nref; d.fixed_cut = 0.0;
+    for (int X = 0; X < kNPairClass; ++X) {
+      d.p0[X] = np_acc[X];
+      d.np[X] = (int)c->pairs[X].size();
+      for (PairRec pr : c->pairs[X]) {
+        pr.a += sub0; pr.b += sub0; pr.pp_off += pp0; pr.inst = i;
+        pairs[X].push_back(pr);
+      }
+      np_acc[X] += d.np[X];
+      const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
+      s_off[i][X] = s_acc;
+      s_acc += (size_t)c->n * ncomp * d.np[X];
+    }
+    mat += (long long)c->n * c->n;
+    pm += (long long)c->nref * c->nref;
+    rows += c->n;
+  }
+  b->mat_total = mat; b->pm_total = pm; b->rows_total = rows; b->s_total = std::max<size_t>(s_acc, 1);
+  int rc;
+  if ((rc = dalloc(&b->d_sub, sub.size())) || (rc = dalloc(&b->d_pp, pp.size())) ||
+      (rc = dalloc(&b->d_bscale, bscale.size())) || (rc = dalloc(&b->d_boys, (size_t)kBoysNT * kBoysCols)) ||
+      (rc = dalloc(&b->d_ref, ref.size())) || (rc = dalloc(&b->d_geo, (size_t)count)) ||
+      (rc = dalloc(&b->d_mat_off, (size_t)count)) || (rc = dalloc(&b->d_pm_off, (size_t)count)) ||
+      (rc = dalloc(&b->d_row0, (size_t)count)) || (rc = dalloc(&b->d_inst, (size_t)count)) ||
+      (rc = dalloc(&b->d_prs, (size_t)kNPairClass)) || (rc = dalloc(&b->d_ncomp, (size_t)kNPairClass)) ||
+      (rc = dalloc(&b->d_nc, (size_t)kNPairClass)))
+    return bail(rc);
+  double** mats[] = {&b->d_DA, &b->d_DB, &b->d_DJ, &b->d_AJ, &b->d_AKA, &b->d_AKB, &b->d_J, &b->d_KA, &b->d_KB};
+  for (double** m : mats)
+    if ((rc = dalloc(m, (size_t)mat))) return bail(rc);
+  if ((rc = dalloc(&b->d_PM, (size_t)std::max(pm, 1LL))) || (rc = dalloc(&b->d_SA, b->s_total)) ||
+      (rc = dalloc(&b->d_SB, b->s_total)))
+    return bail(rc);
+  const PairRec* prs[kNPairClass];
+  int ncomp[kNPairClass], nc[kNPairClass];
+  for (int X = 0; X < kNPairClass; ++X) {
+    b->np_total[X] = (int)pairs[X].size();
+    const int nab = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]);
+    b->jp_len[X] = pairs[X].size() * nab;
+    if ((rc = dalloc(&b->d_pairs[X], pairs[X].size()))) return bail(rc);
+    if (b->jp_len[X] && (rc = dalloc(&b->d_Jp[X], b->jp_len[X]))) return bail(rc);
+    if (!pairs[X].empty())
+      HIPCHK(hipMemcpyAsync(b->d_pairs[X], pairs[X].data(), pairs[X].size() * sizeof(PairRec), hipMemcpyHostToDevice,
+                            b->st));
+    prs[X] = b->d_pairs[X];
+    nc[X] = sqmd::ncart(kPairL[X][0]);
+    ncomp[X] = nc[X] + sqmd::ncart(kPairL[X][1]);
+  }
+  for (int i = 0; i < count; ++i) {
+    InstDesc& d = b->desc[i];
+    const BatchGeom& g = b->geo[i];
+    d.DJ = b->d_DJ + g.mat_off; d.DA = b->d_DA + g.mat_off; d.DB = b->d_DB + g.mat_off;
+    d.PM = b->d_PM + g.pm_off;
+    d.AJ = b->d_AJ + g.mat_off; d.AKA = b->d_AKA + g.mat_off; d.AKB = b->d_AKB + g.mat_off;
+    for (int X = 0; X < kNPairClass; ++X) {
+      d.SA[X] = b->d_SA + s_off[i][X];
+      d.SB[X] = b->uhf[i] ? b->d_SB + s_off[i][X] : nullptr;
+    }
+  }
+  const std::vector<double>& boys = boys_table();
+  HIPCHK(hipMemcpyAsync(b->d_sub, sub.data(), sub.size() * sizeof(SubShell), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_pp, pp.data(), pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_bscale, bscale.data(), bscale.size() * sizeof(double), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_boys, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_ref, ref.data(), ref.size() * sizeof(int), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_geo, b->geo.data(), count * sizeof(BatchGeom), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_mat_off, mat_off.data(), count * sizeof(long long), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_pm_off, pm_off.data(), count * sizeof(long long), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_row0, row0.data(), count * sizeof(long long), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_prs, prs, sizeof(prs), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_ncomp, ncomp, sizeof(ncomp), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_nc, nc, sizeof(nc), hipMemcpyHostToDevice, b->st));
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(mat, 256)), dim3(256), 0, b->st, b->d_DA, (size_t)mat, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(mat, 256)), dim3(256), 0, b->st, b->d_DB, (size_t)mat, 0.0);
+  HIPCHK(hipStreamSynchronize(b->st));
+  *out = b;
+  return SQJK_OK;
+}
+
+int sqjk_batch_set_density(sqjk_batch* b, int i, const double* PA, const double* PB, int on_device) {
+  if (!b || i < 0 || i >= b->count || !PA || !PB) return fail(SQJK_EARG, "sqjk_batch_set_density: invalid arguments");
+  HIPCHK(hipSetDevice(b->device));
+  const BatchGeom& g = b->geo[i];
+  const size_t bytes = (size_t)g.n * g.n * sizeof(double);
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(b->d_DA + g.mat_off, PA, bytes, k, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_DB + g.mat_off, b->uhf[i] ? PB : PA, bytes, k, b->st));
+  if (!on_device) HIPCHK(hipStreamSynchronize(b->st));  // host arrays may be reused at once
+  return SQJK_OK;
+}
+
+int sqjk_batch_set_cutoffs(sqjk_batch* b, int i, double fixedCutoff, double scfCutoff) {
+  if (!b || i < -1 || i >= b->count) return fail(SQJK_EARG, "sqjk_batch_set_cutoffs: invalid arguments");
+  for (int k = (i < 0 ? 0 : i); k < (i < 0 ? b->count : i + 1); ++k) {
+    if (b->fixed[k] != fixedCutoff || b->scf[k] != scfCutoff) b->plan_ok = false;
+    b->fixed[k] = fixedCutoff;
+    b->scf[k] = scfCutoff;
+  }
+  return SQJK_OK;
+}
+
+int sqjk_batch_set_mixed(sqjk_batch* b, double threshold) {
+  if (!b || threshold < 0.0) return fail(SQJK_EARG, "sqjk_batch_set_mixed: invalid arguments");
+  if (threshold != b->mixed_thr) b->plan_ok = false;
+  b->mixed_thr = threshold;
+  return SQJK_OK;
+}
+
+int sqjk_batch_set_tuning(sqjk_batch* b, const int* rows, const int* split_budget, const int* grid_over) {
+  if (!b) return fail(SQJK_EARG, "sqjk_batch_set_tuning: null batch");
+  for (sqjk_ctx* c : b->ctx)
+    if (int rc = sqjk_set_tuning(c, rows, split_budget, grid_over)) return rc;
+  b->plan_ok = false;
+  return SQJK_OK;
+}
+
+int sqjk_batch_set_profile(sqjk_batch* b, int on) {
+  if (!b) return fail(SQJK_EARG, "sqjk_batch_set_profile: null batch");
+  if ((on != 0) != b->profile) b->plan_ok = false;  // the graph holds the events or not
+  b->profile = on != 0;
+  return SQJK_OK;
+}
+
+int sqjk_batch_build(sqjk_batch* b, int use_graph) {
+  if (!b) return fail(SQJK_EARG, "sqjk_batch_build: null batch");
+  HIPCHK(hipSetDevice(b->device));
+  int rc;
+  if (!b->plan_ok && (rc = batch_plan(b))) return rc;
+  const bool prof = b->profile && !use_graph;
+  if (prof && b->prof_ev.size() < 2 * b->launches.size()) {
+    const size_t have = b->prof_ev.size();
+    b->prof_ev.resize(2 * b->launches.size());
+    for (size_t e = have; e < b->prof_ev.size(); ++e) HIPCHK(hipEventCreate(&b->prof_ev[e]));
+  }
+  HIPCHK(hipEventRecord(b->ev0, b->st));
+  if (!use_graph || !b->exec) {
+    // eager; the first graph build of a plan also runs eagerly (the
+    // launchers size their grids from device queries then) and captures the
+    // graph that the next builds replay
+    if ((rc = batch_enqueue(b, prof))) return rc;
+  } else {
+    HIPCHK(hipGraphLaunch(b->exec, b->st));
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  std::memset(b->cls_ms, 0, sizeof(b->cls_ms));
+  if (prof)
+    for (size_t li = 0; li < b->launches.size(); ++li) {
+      float lm = 0.f;
+      HIPCHK(hipEventElapsedTime(&lm, b->prof_ev[2 * li], b->prof_ev[2 * li + 1]));
+      b->cls_ms[b->launches[li].X][b->launches[li].Y] += lm;
+    }
+  if (use_graph && !b->exec) {
+    HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(SQJK_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  return SQJK_OK;
+}
+
+int sqjk_batch_result(sqjk_batch* b, int i, double* J, double* KA, double* KB, int on_device) {
+  if (!b || i < 0 || i >= b->count) return fail(SQJK_EARG, "sqjk_batch_result: invalid arguments");
+  HIPCHK(hipSetDevice(b->device));
+  const BatchGeom& g = b->geo[i];
+  const size_t bytes = (size_t)g.n * g.n * sizeof(double);
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (J) HIPCHK(hipMemcpyAsync(J, b->d_J + g.mat_off, bytes, k, b->st));
+  if (KA) HIPCHK(hipMemcpyAsync(KA, b->d_KA + g.mat_off, bytes, k, b->st));
+  if (KB) HIPCHK(hipMemcpyAsync(KB, b->d_KB + g.mat_off, bytes, k, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return SQJK_OK;
+}
+
+int sqjk_batch_stats(const sqjk_batch* b, double* kernel_ms, long long* quartets, int* launches,
+                     double* class_ms) {
+  if (!b) return fail(SQJK_EARG, "sqjk_batch_stats: null batch");
+  if (kernel_ms) *kernel_ms = b->last_ms;
+  if (quartets) *quartets = b->plan_q;
+  if (launches) *launches = b->last_launches;
+  if (class_ms) {
+    int k = 0;
+    for (int X = 0; X < kNPairClass; ++X)
+      for (int Y = 0; Y <= X; ++Y, ++k) class_ms[k] = b->cls_ms[X][Y];
+  }
+  return SQJK_OK;
+}
+
+void sqjk_batch_destroy(sqjk_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_free_plan(b);
+  void* ptrs[] = {b->d_sub, b->d_pp, b->d_bscale, b->d_boys, b->d_ref, b->d_geo, b->d_mat_off, b->d_pm_off,
+                  b->d_row0, b->d_inst, b->d_prs, b->d_ncomp, b->d_nc, b->d_DA, b->d_DB, b->d_DJ, b->d_PM,
+                  b->d_AJ, b->d_AKA, b->d_AKB, b->d_J, b->d_KA, b->d_KB, b->d_SA, b->d_SB};
+  for (void* p : ptrs) block_put(p);
+  for (int X = 0; X < kNPairClass; ++X) {
+    block_put(b->d_pairs[X]);
+    block_put(b->d_Jp[X]);
+  }
+  for (hipEvent_t e : b->prof_ev) (void)hipEventDestroy(e);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+}  // extern "C"
 
 // ------------------------------------------------------------ drop-ins
 static sqjk_ctx* g_ctx = nullptr;  // the reference keeps this state in statics too

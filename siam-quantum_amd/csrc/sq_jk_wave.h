@@ -105,7 +105,6 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
   // (runtime loops over them keep the register count low).
   R* acc = reinterpret_cast<R*>(lds + W_::oAcc);
   const int lane = threadIdx.x;
-  const int n = A.n;
 
   // Hermite index decode table in LDS (t | u << 8 | v << 16), and per lane
   // the recursion step of each R entry it owns, e = lane + kWT s:
@@ -138,15 +137,17 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
   for (long long t = blockIdx.x; t < items; t += gridDim.x) {
     // ket primitive pairs [v0, v1): the whole range in Schwarz mode, the
     // tile's share of a segment split along its ket primitive pairs otherwise
-    int r0 = 0, r1 = 1, j = 0, nppb = 0, v0 = 0, v1 = 0, ket0 = 0;
+    int r0 = 0, r1 = 1, j = 0, nppb = 0, v0 = 0, v1 = 0, ket0 = 0, inst = 0;
     if constexpr (MODE == kSchwarz) {
       nppb = v1 = uload(A.bra + t).npp;
     } else {
       const Tile tl = uload(A.tile + (t >> 6));
       const Segment sg = uload(A.seg + tl.seg);
       r0 = tl.r0; r1 = tl.r1; j = tl.k0 + (int)(t & 63);
-      nppb = sg.nppb; v0 = tl.v0; v1 = tl.v1; ket0 = sg.ket0;
+      nppb = sg.nppb; v0 = tl.v0; v1 = tl.v1; ket0 = sg.ket0; inst = sg.inst;
     }
+    const View V = make_view(A, inst);
+    const int n = V.n;
     for (int r = r0; r < r1; ++r) {
     int ib, ik;
     if constexpr (MODE == kSchwarz) {
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
       if (j < uload(A.rlo + r) || j >= uload(A.rhi + r)) continue;  // uniform
       ib = uload(A.rbra + r);
       ik = ket0 + j;
-      if (!density_screen_pass(A, uload(A.bra + ib), uload(A.ket + ik))) continue;  // uniform
+      if (!density_screen_pass(V, uload(A.bra + ib), uload(A.ket + ik))) continue;  // uniform
     }
     const PairRec bp = uload(A.bra + ib), kp = uload(A.ket + ik);
     const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = uload(A.sh + kp.a), sD = uload(A.sh + kp.b);
@@ -258,13 +259,14 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
     }
 
     const int fa = sA.bf_off, fb = sB.bf_off, fc = sC.bf_off, fd = sD.bf_off;
+    const int ga = sA.gbf, gb = sB.gbf, gc = sC.gbf, gd = sD.gbf;  // component scale indices
     if constexpr (MODE == kSchwarz) {
       for (int o = lane; o < NINT; o += kWT) {
         {
           const int ab = o / NCD, cd = o - ab * NCD;
           if (ab == cd) {
             const int ia = ab / NB, jb = ab % NB;
-            const double sc = A.bscale[fa + ia] * A.bscale[fb + jb];
+            const double sc = A.bscale[ga + ia] * A.bscale[gb + jb];
             const double v = sqrt(fabs((double)acc[o] * sc * sc));
             A.sch_out[(fa + ia) * n + fb + jb] = v;
             A.sch_out[(fb + jb) * n + fa + ia] = v;
@@ -280,34 +282,34 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
       for (int o = lane; o < NINT; o += kWT) {
         {
           const int ab = o / NCD, cd = o - ab * NCD;
-          EE[o] = (double)acc[o] * deg * A.bscale[fa + ab / NB] * A.bscale[fb + ab % NB] *
-                  A.bscale[fc + cd / ND] * A.bscale[fd + cd % ND];
+          EE[o] = (double)acc[o] * deg * A.bscale[ga + ab / NB] * A.bscale[gb + ab % NB] *
+                  A.bscale[gc + cd / ND] * A.bscale[gd + cd % ND];
         }
       }
       wave_sync();
       // Coulomb
       double* Dst = lds + W_::oD;
-      for (int e = lane; e < NAB; e += kWT) Dst[e] = A.DJ[(fa + e / NB) * n + fb + e % NB];
-      for (int e = lane; e < NCD; e += kWT) Dst[NAB + e] = A.DJ[(fc + e / ND) * n + fd + e % ND];
+      for (int e = lane; e < NAB; e += kWT) Dst[e] = V.DJ[(fa + e / NB) * n + fb + e % NB];
+      for (int e = lane; e < NCD; e += kWT) Dst[NAB + e] = V.DJ[(fc + e / ND) * n + fd + e % ND];
       wave_sync();
       for (int e = lane; e < NAB + NCD; e += kWT) {
         double s = 0.0;
         if (e < NAB) {
 #pragma unroll 6
           for (int cd = 0; cd < NCD; ++cd) s += EE[e * NCD + cd] * Dst[NAB + cd];
-          fatomic(&A.AJ[(fa + e / NB) * n + fb + e % NB], 2.0 * s);
+          fatomic(&V.AJ[(fa + e / NB) * n + fb + e % NB], 2.0 * s);
         } else {
           const int cd = e - NAB;
 #pragma unroll 6
           for (int ab = 0; ab < NAB; ++ab) s += EE[ab * NCD + cd] * Dst[ab];
-          fatomic(&A.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * s);
+          fatomic(&V.AJ[(fc + cd / ND) * n + fd + cd % ND], 2.0 * s);
         }
       }
       // Exchange, per spin
       constexpr int nspin = (MODE == kUHF) ? 2 : 1;
       for (int sp = 0; sp < nspin; ++sp) {
-        const double* D = sp ? A.DB : A.DA;
-        double* AK = sp ? A.AKB : A.AKA;
+        const double* D = sp ? V.DB : V.DA;
+        double* AK = sp ? V.AKB : V.AKA;
         double* dbd = Dst + NAB + NCD;        // D[b,d]  NB x ND
         double* dad = dbd + NB * ND;          // D[a,d]  NA x ND
         double* dbc = dad + NA * ND;          // D[b,c]  NB x NC

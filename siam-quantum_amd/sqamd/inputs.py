@@ -231,3 +231,97 @@ def c60(r_single=1.45, r_double=1.40):
     ls, ld = lengths(geom(t, 1.0))
     X = geom(t, r_single / ls)
     return [6] * 60, X - X.mean(axis=0)
+
+
+# ------------------------------------------------ small-molecule instances
+def _place(center, direction, r):
+    d = np.asarray(direction, dtype=float)
+    return np.asarray(center, dtype=float) + r * d / np.linalg.norm(d)
+
+
+def small_molecule(name):
+    """Equilibrium-like geometries (angstrom) of the small molecules of the
+    batched workload, built from standard bond lengths and angles:
+    H2O, NH3, CH4, C2H2, C2H4, C2H6, CH2O, CH3OH, C6H6."""
+    t = np.deg2rad
+    if name == "H2O":
+        return water_c1()
+    if name == "NH3":  # N-H 1.012, H-N-H 106.7
+        r, a = 1.012, t(106.7)
+        # angle from the C3 axis for a given H-N-H angle
+        th = np.arcsin(np.sqrt((1 - np.cos(a)) / 1.5))
+        H = [[r * np.sin(th) * np.cos(k * 2 * np.pi / 3), r * np.sin(th) * np.sin(k * 2 * np.pi / 3),
+              -r * np.cos(th)] for k in range(3)]
+        return [7, 1, 1, 1], np.array([[0, 0, 0]] + H)
+    if name == "CH4":  # C-H 1.087, tetrahedral
+        d = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / np.sqrt(3) * 1.087
+        return [6, 1, 1, 1, 1], np.vstack([[0, 0, 0], d])
+    if name == "C2H2":  # C-C 1.203, C-H 1.063
+        return [6, 6, 1, 1], np.array([[0, 0, 0.6015], [0, 0, -0.6015], [0, 0, 1.6645], [0, 0, -1.6645]])
+    if name == "C2H4":  # C=C 1.339, C-H 1.086, H-C-H 117.6
+        h = t(117.6) / 2
+        z, hx, hz = 0.6695, 1.086 * np.sin(h), 1.086 * np.cos(h)
+        return [6, 6, 1, 1, 1, 1], np.array([[0, 0, z], [0, 0, -z], [hx, 0, z + hz], [-hx, 0, z + hz],
+                                             [hx, 0, -z - hz], [-hx, 0, -z - hz]])
+    if name == "C2H6":  # C-C 1.535, C-H 1.094, H-C-C 111.2, staggered
+        z, r, a = 0.7675, 1.094, t(180 - 111.2)
+        H = []
+        for s, ph0 in ((1, 0.0), (-1, np.pi / 3)):
+            for k in range(3):
+                ph = ph0 + k * 2 * np.pi / 3
+                H.append([r * np.sin(a) * np.cos(ph), r * np.sin(a) * np.sin(ph), s * (z + r * np.cos(a))])
+        return [6, 6] + [1] * 6, np.array([[0, 0, z], [0, 0, -z]] + H)
+    if name == "CH2O":  # C=O 1.205, C-H 1.111, H-C-H 116.1
+        h = t(116.1) / 2
+        return [6, 8, 1, 1], np.array([[0, 0, 0], [0, 0, 1.205], [1.111 * np.sin(h), 0, -1.111 * np.cos(h)],
+                                       [-1.111 * np.sin(h), 0, -1.111 * np.cos(h)]])
+    if name == "CH3OH":  # C-O 1.427, O-H 0.956, C-H 1.094, C-O-H 108.9
+        a = t(108.9)
+        O = np.array([0, 0, 1.427])
+        Ho = O + 0.956 * np.array([np.sin(np.pi - a), 0, np.cos(np.pi - a)])
+        Z, X = small_molecule("CH4")
+        Hs = [X[2], X[3], X[4]]  # three of the tetrahedral directions, rotated to point away from O
+        R = np.eye(3)
+        v = X[1] / np.linalg.norm(X[1])
+        # rotate the tetrahedron so that the first C-H direction points along +z (taken by O)
+        w = np.cross(v, [0, 0, 1])
+        s, c = np.linalg.norm(w), np.dot(v, [0, 0, 1])
+        if s > 1e-12:
+            k = w / s
+            Kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+            R = np.eye(3) + s * Kx + (1 - c) * Kx @ Kx
+        Hs = [R @ h for h in Hs]
+        return [6, 8, 1, 1, 1, 1], np.vstack([[0, 0, 0], O, Ho, Hs])
+    if name == "C6H6":  # C-C 1.397, C-H 1.084
+        C = [[1.397 * np.cos(k * np.pi / 3), 1.397 * np.sin(k * np.pi / 3), 0] for k in range(6)]
+        H = [[2.481 * np.cos(k * np.pi / 3), 2.481 * np.sin(k * np.pi / 3), 0] for k in range(6)]
+        return [6] * 6 + [1] * 6, np.array(C + H)
+    raise ValueError(name)
+
+
+BATCH_MOLECULES = ["H2O", "NH3", "CH4", "C2H2", "C2H4", "C2H6", "CH2O", "CH3OH", "C6H6"]
+BATCH_BASES = ["sto3g", "631g", "631g_star"]
+
+
+def batch_instances(count=256, seed=4256, uhf_fraction=0.25, sigma=0.05):
+    """Seeded instances of the batched workload: molecule and basis drawn
+    uniformly from BATCH_MOLECULES x BATCH_BASES (7 to 102 basis
+    functions), every coordinate displaced by N(0, sigma) angstrom, and a
+    fraction turned into UHF doublet cations (charge +1, multiplicity 2).
+    Returns dicts with name, basis, mol (Mol, bohr), charge, nEA, nEB,
+    restricted."""
+    rng = np.random.default_rng(seed)
+    nuhf = int(round(uhf_fraction * count))
+    uhf = set(rng.choice(count, nuhf, replace=False).tolist())
+    out = []
+    for i in range(count):
+        name = BATCH_MOLECULES[rng.integers(len(BATCH_MOLECULES))]
+        basis = BATCH_BASES[rng.integers(len(BATCH_BASES))]
+        Z, X = small_molecule(name)
+        X = np.asarray(X, dtype=float) + rng.normal(0.0, sigma, np.shape(X))
+        charge = 1 if i in uhf else 0
+        ne = int(np.sum(Z)) - charge
+        mult = 2 if i in uhf else 1
+        out.append({"name": name, "basis": basis, "mol": mol_from_angstrom(Z, X, charge), "charge": charge,
+                    "nEA": (ne + mult - 1) // 2, "nEB": (ne - mult + 1) // 2, "restricted": i not in uhf})
+    return out
