@@ -215,6 +215,44 @@ def batch_bench(steps, mixed_thr, cut=1e-10):
     return out
 
 
+def serve_bench(count, rate, mixed_thr, slots):
+    """Config C5: Poisson arrivals of RHF/UHF 6-31G requests (random C/H/N/O
+    chains, 10-200 atoms log-uniform, two geometry steps each), served with
+    device-resident SCF state, admission by device-memory footprint.  The same
+    arrival trace runs in FP64 and in mixed precision."""
+    from sqamd import inputs
+    from sqamd.serve import serve
+    reqs = inputs.serving_requests(count=count, rate=rate, seed=5005, steps=2)
+    out = {"workload": f"{count} requests, Poisson arrivals at {rate:g}/s (seed 5005), random C/H/N/O chains "
+                       f"of {min(r['natom'] for r in reqs)}-{max(r['natom'] for r in reqs)} atoms, 6-31G, "
+                       f"2 geometry steps (sigma 0.05 A) each, SCFConv 1e-6, 3-step cut-offs, DIIS4",
+           "slots": slots, "uhf_requests": int(sum(r["multiplicity"] == 2 for r in reqs))}
+    E = {}
+    for label, thr in (("fp64", 0.0), ("mixed", mixed_thr)):
+        if label == "mixed" and thr <= 0:
+            continue
+        t0 = time.perf_counter()
+        res = serve(reqs, slots=slots, mixed=thr, conv=1e-6, mem_budget=96e9)
+        wall = time.perf_counter() - t0
+        lat = np.array([st["finish"] - st["arrival"] for _, _, st in res])
+        solves = sum(len(e) for e, _, _ in res)
+        conv = sum(st["converged_steps"] for _, _, st in res)
+        makespan = max(st["finish"] for _, _, st in res)
+        E[label] = np.concatenate([e for e, _, _ in res])
+        out[label] = {"scf_solves_per_s": solves / makespan, "requests_per_s": len(res) / makespan,
+                      "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                      "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                      "converged_steps": int(conv), "steps": int(solves), "makespan_s": makespan, "wall_s": wall,
+                      "peak_in_flight": res[0][2]["peak_in_flight"],
+                      "peak_memory_GB": res[0][2]["peak_memory"] / 1e9,
+                      "basis_functions": [int(st["nBasis"]) for _, _, st in res],
+                      "iterations": int(sum(st["iterations"] for _, _, st in res))}
+    if "mixed" in E:
+        ok = (E["fp64"] != 0) & (E["mixed"] != 0)
+        out["mixed"]["max_abs_dE_vs_fp64"] = float(np.abs(E["mixed"][ok] - E["fp64"][ok]).max()) if ok.any() else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +266,9 @@ def main():
     ap.add_argument("--scf-solves", type=int, default=100, help="warm SCF solves per precision (0: skip)")
     ap.add_argument("--scf-workload", default="w16", choices=["w16", "w32"])
     ap.add_argument("--no-extras", action="store_true", help="only the headline C60 build line")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -344,6 +385,8 @@ def main():
                       "value": a.steps / w32, "unit": "J/K builds/s", "kernel_ms": k32}
         print("bench: batched builds", file=sys.stderr, flush=True)
         out["batch"] = batch_bench(a.steps, a.mixed_threshold)
+        print("bench: serving", file=sys.stderr, flush=True)
+        out["serve"] = serve_bench(a.serve_requests, a.serve_rate, a.mixed_threshold, a.serve_slots)
         if a.scf_solves > 0:
             print("bench: SCF solves", file=sys.stderr, flush=True)
             out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)

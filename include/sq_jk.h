@@ -52,6 +52,13 @@ typedef struct {
 int sqjk_create(int nBasis, const struct GTOBasis_t *gto, const double *schwarz_basis,
                 int device, sqjk_ctx **out);
 
+/* Device memory a J/K context for this basis takes (pair and primitive
+ * data, matrices, exchange staging for RHF or UHF), plus that of a
+ * device-resident SCF when with_scf is set; an estimate from the shell and
+ * pair counts, computed on the host without touching the device.  Used to
+ * admit served requests by HBM footprint.  Negative on error. */
+long long sqjk_footprint_bytes(int nBasis, const struct GTOBasis_t *gto, int uhf, int with_scf);
+
 /* Moves the context to a new geometry of the same basis (same functions,
  * exponents and coefficients; only the centres differ), as in the
  * geometry steps of an optimisation or molecular dynamics run

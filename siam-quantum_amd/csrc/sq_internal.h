@@ -9,3 +9,7 @@ struct sqjk_ctx;
 hipStream_t sqjk_stream_internal(sqjk_ctx* c);
 // mixed-precision threshold set for the drop-in entry points
 double sqjk_dropin_mixed_internal();
+// device blocks from the library's cache (no hipFree per release, which
+// would wait for the whole device)
+void* sq_device_block(size_t bytes);
+void sq_device_release(void* p);

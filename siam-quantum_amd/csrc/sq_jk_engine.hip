@@ -913,6 +913,26 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   return SQJK_OK;
 }
 
+long long sqjk_footprint_bytes(int nBasis, const GTOBasis_t* gto, int uhf, int with_scf) {
+  if (nBasis <= 0 || !gto) return fail(SQJK_EARG, "sqjk_footprint_bytes: invalid arguments");
+  sqjk_ctx t;
+  std::vector<double> pexp, pcoef;
+  if (int rc = plan_shells(&t, nBasis, gto, pexp, pcoef)) return rc;
+  plan_pairs(&t, pexp, pcoef);
+  const long long nn = (long long)nBasis * nBasis;
+  long long b = (long long)t.sub.size() * sizeof(SubShell) + (long long)t.pp.size() * sizeof(PrimPair) +
+                (long long)kBoysNT * kBoysCols * 8 + (long long)nBasis * 8 + (long long)t.nref * t.nref * 8;
+  b += 10 * nn * 8;  // densities, half-accumulators, results, Schwarz
+  for (int X = 0; X < kNPairClass; ++X) {
+    const long long np = (long long)t.pairs[X].size();
+    const int nab = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]);
+    const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
+    b += np * sizeof(PairRec) + np * nab * 8 + (uhf ? 2 : 1) * (long long)nBasis * ncomp * np * 8;
+  }
+  if (with_scf) b += 38 * nn * 8;  // SCF matrices, DIIS history, eigen-solver work
+  return b + b / 8;                 // work plan and allocator slack
+}
+
 int sqjk_update_geometry(sqjk_ctx* c, const GTOBasis_t* gto) {
   if (!c || !gto) return fail(SQJK_EARG, "sqjk_update_geometry: null argument");
   HIPCHK(hipSetDevice(c->device));
@@ -975,15 +995,27 @@ int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, 
     atoms[4 * k + 3] = (double)mol->Z[k];
   }
   const size_t nn = (size_t)nBasis * nBasis;
+  // a stream per host thread, so that one-electron matrices of molecules
+  // served side by side do not wait for each other's work
+  thread_local hipStream_t st = nullptr;
+  thread_local int st_dev = -1;
+  if (!st || st_dev != device) {
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    st_dev = device;
+  }
   std::vector<void*> bufs;
+  bool up_ok = true;
   auto up = [&](const void* h, size_t bytes) -> void* {
     void* d = nullptr;
-    if (!(d = block_get(std::max<size_t>(bytes, 8), false))) return nullptr;
+    if (!(d = block_get(std::max<size_t>(bytes, 8), false))) { up_ok = false; return nullptr; }
     bufs.push_back(d);
-    if (h && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    if (h && bytes && hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) != hipSuccess) up_ok = false;
     return d;
   };
-  auto release = [&] { for (void* d : bufs) block_put(d); };
+  auto release = [&] {
+    (void)hipStreamSynchronize(st);
+    for (void* d : bufs) block_put(d);
+  };
   const std::vector<double>& boys = boys_table();
   OneEArgs a{};
   a.sh = (const SubShell*)up(c.sub.data(), c.sub.size() * sizeof(SubShell));
@@ -997,22 +1029,23 @@ int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, 
   a.S = S ? (double*)up(nullptr, nn * 8) : nullptr;
   a.natom = mol->nAtom;
   a.n = nBasis;
-  if (!a.sh || !a.pexp || !a.pcoef || !a.bscale || !a.atom || !a.boys || (T && !a.T) || (V && !a.V) || (S && !a.S)) {
+  if (!up_ok) {
     release();
     return fail(SQJK_ENOMEM, "sqjk_one_electron: device allocation or upload failed");
   }
   for (int X = 0; X < kNPairClass; ++X) {
     a.pr = (const PairRec*)up(c.pairs[X].data(), c.pairs[X].size() * sizeof(PairRec));
     a.np = (int)c.pairs[X].size();
-    if (!a.pr || launch_one_electron(X, a, 0) != 0) {
+    if (!up_ok || launch_one_electron(X, a, st) != 0) {
       release();
       return fail(SQJK_EHIP, "sqjk_one_electron: kernel launch failed");
     }
   }
-  hipError_t e = hipDeviceSynchronize();
-  if (e == hipSuccess && T) e = hipMemcpy(T, a.T, nn * 8, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && V) e = hipMemcpy(V, a.V, nn * 8, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && S) e = hipMemcpy(S, a.S, nn * 8, hipMemcpyDeviceToHost);
+  hipError_t e = hipSuccess;
+  if (T) e = hipMemcpyAsync(T, a.T, nn * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && V) e = hipMemcpyAsync(V, a.V, nn * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && S) e = hipMemcpyAsync(S, a.S, nn * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   release();
   if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_one_electron: ") + hipGetErrorString(e));
   return SQJK_OK;
@@ -2056,6 +2089,8 @@ static double g_dropin_mixed = 0.0;
 }  // extern "C"
 
 hipStream_t sqjk_stream_internal(sqjk_ctx* c) { return c ? c->st : nullptr; }
+void* sq_device_block(size_t bytes) { return block_get(bytes, false); }
+void sq_device_release(void* p) { block_put(p); }
 double sqjk_dropin_mixed_internal() { return g_dropin_mixed; }
 
 extern "C" {

@@ -240,13 +240,13 @@ void free_mats(sqscf_ctx* c) {
   double** m[] = {&c->H, &c->S, &c->PA, &c->PB, &c->PoldA, &c->PoldB, &c->dPA, &c->dPB, &c->JT, &c->KA,
                   &c->KB, &c->Jd, &c->KAd, &c->KBd, &c->FA, &c->FB, &c->CA, &c->CB, &c->eA, &c->eB};
   for (double** p : m) {
-    if (*p) (void)hipFree(*p);
+    sq_device_release(*p);
     *p = nullptr;
   }
   for (int s = 0; s < kMaxHist; ++s)
     for (int k = 0; k < 2; ++k) {
-      if (c->hp[s][k]) (void)hipFree(c->hp[s][k]);
-      if (c->he[s][k]) (void)hipFree(c->he[s][k]);
+      sq_device_release(c->hp[s][k]);
+      sq_device_release(c->he[s][k]);
       c->hp[s][k] = c->he[s][k] = nullptr;
     }
   c->cap = 0;
@@ -258,13 +258,14 @@ int alloc_mats(sqscf_ctx* c, int n) {
   const size_t nn = (size_t)n * n;
   double** m[] = {&c->H, &c->S, &c->PA, &c->PB, &c->PoldA, &c->PoldB, &c->dPA, &c->dPB, &c->JT,
                   &c->KA, &c->KB, &c->Jd, &c->KAd, &c->KBd, &c->FA, &c->FB, &c->CA, &c->CB};
+  auto get = [&](double** p, size_t bytes) { return (*p = static_cast<double*>(sq_device_block(bytes))) != nullptr; };
   for (double** p : m)
-    if (hipMalloc(p, nn * 8) != hipSuccess) return scf_fail(SQJK_ENOMEM, "SCF device allocation failed");
-  if (hipMalloc(&c->eA, (size_t)n * 8) != hipSuccess || hipMalloc(&c->eB, (size_t)n * 8) != hipSuccess)
+    if (!get(p, nn * 8)) return scf_fail(SQJK_ENOMEM, "SCF device allocation failed");
+  if (!get(&c->eA, (size_t)n * 8) || !get(&c->eB, (size_t)n * 8))
     return scf_fail(SQJK_ENOMEM, "SCF device allocation failed");
   for (int s = 0; s < kMaxHist; ++s)
     for (int k = 0; k < 2; ++k)
-      if (hipMalloc(&c->hp[s][k], nn * 8) != hipSuccess || hipMalloc(&c->he[s][k], nn * 8) != hipSuccess)
+      if (!get(&c->hp[s][k], nn * 8) || !get(&c->he[s][k], nn * 8))
         return scf_fail(SQJK_ENOMEM, "SCF device allocation failed");
   if (!c->partial && (hipMalloc(&c->partial, kRedBlocks * kMaxSums * 8) != hipSuccess ||
                       hipMalloc(&c->sums, kMaxSums * 8) != hipSuccess))

@@ -325,3 +325,76 @@ def batch_instances(count=256, seed=4256, uhf_fraction=0.25, sigma=0.05):
         out.append({"name": name, "basis": basis, "mol": mol_from_angstrom(Z, X, charge), "charge": charge,
                     "nEA": (ne + mult - 1) // 2, "nEB": (ne - mult + 1) // 2, "restricted": i not in uhf})
     return out
+
+
+# ------------------------------------------------------- served requests
+def random_chain(natom, rng, bond=(1.0, 1.5), h_bond=(1.0, 1.1), min_heavy=2.0, min_any=1.5):
+    """A self-avoiding chain of heavy atoms (C 70 %, N 15 %, O 15 %) with
+    bond lengths U(bond) angstrom, hydrogens on the chain atoms at U(h_bond)
+    (at most three per atom); non-bonded heavy atoms at least `min_heavy`
+    apart and every other non-bonded pair at least `min_any`.  About 60 % of
+    the atoms are heavy.  Returns (Z, xyz angstrom)."""
+    nheavy = max(1, int(round(0.6 * natom)))
+    nh = natom - nheavy
+    for _attempt in range(200):
+        Z = [int(rng.choice([6, 7, 8], p=[0.7, 0.15, 0.15])) for _ in range(nheavy)]
+        X = [np.zeros(3)]
+        prev_dir = None
+        ok = True
+        for k in range(1, nheavy):
+            for _try in range(200):
+                d = rng.standard_normal(3)
+                d /= np.linalg.norm(d)
+                if prev_dir is not None and np.dot(d, prev_dir) < 0.0:  # keep the chain from folding back
+                    continue
+                p = X[-1] + d * rng.uniform(*bond)
+                if all(np.linalg.norm(p - q) >= min_heavy for q in X[:-1]):
+                    X.append(p)
+                    prev_dir = d
+                    break
+            else:
+                ok = False
+                break
+        if not ok:
+            continue
+        count = [0] * nheavy
+        H = []
+        for _ in range(nh):
+            for _try in range(500):
+                a = int(rng.integers(nheavy))
+                if count[a] >= 3:
+                    continue
+                d = rng.standard_normal(3)
+                d /= np.linalg.norm(d)
+                p = X[a] + d * rng.uniform(*h_bond)
+                if all(np.linalg.norm(p - q) >= min_any for i, q in enumerate(X) if i != a) and \
+                        all(np.linalg.norm(p - q) >= min_any for q in H):
+                    H.append(p)
+                    count[a] += 1
+                    break
+            else:
+                ok = False
+                break
+        if ok:
+            return Z + [1] * nh, np.array(X + H)
+    raise RuntimeError("random_chain: placement failed")
+
+
+def serving_requests(count=16, rate=4.0, seed=5005, steps=2, atoms=(10, 200), sigma=0.05):
+    """Seeded requests of the serving workload: atom counts log-uniform in
+    `atoms`, random C/H/N/O chains (random_chain), `steps` geometry steps
+    each displaced by N(0, sigma) angstrom from the base geometry, Poisson
+    arrivals at `rate` per second.  RHF for an even electron count, else a
+    UHF doublet.  Returns dicts with Z, xyz_bohr [steps, natom, 3], charge,
+    multiplicity, arrival (s), natom."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate) if i > 0 else 0.0
+        natom = int(round(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1])))))
+        Z, X = random_chain(natom, rng)
+        geo = np.stack([(X + rng.normal(0.0, sigma, X.shape)) * ANGSTROM2BOHR for _ in range(steps)])
+        ne = int(np.sum(Z))
+        out.append({"Z": np.array(Z, dtype=np.int32), "xyz_bohr": geo, "charge": 0,
+                    "multiplicity": 1 if ne % 2 == 0 else 2, "arrival": t, "natom": natom})
+    return out
