@@ -141,6 +141,27 @@ class Oracle:
         return E, it.value
 
 
+class ReferenceCheck:
+    """The reference's checkpoint reader/writer (src/check.c, compiled into
+    oracle/_ref/libsqref_check.so).  Loaded with lazy binding: its one
+    unresolved symbol, linear_solver, is only reached through
+    guess_checkpoint_orbital, which is not bound here."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libsqref_check.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+        L = self.L = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        L.save_checkpoint.argtypes = [I, PG, PM, D, D, D, PD, PD, PD, PD, C.c_void_p]
+        L.load_checkpoint_orbital.argtypes = [I, PD, PD, PD, PD, C.c_void_p]
+        L.load_checkpoint_orbital.restype = D
+        L.load_checkpoint_molecule.argtypes = [C.c_void_p]
+        L.load_checkpoint_molecule.restype = PM
+        L.load_checkpoint_basis.argtypes = [C.c_void_p, C.POINTER(I)]
+        L.load_checkpoint_basis.restype = PG
+        L.load_checkpoint_multiplicity.argtypes = [C.c_void_p]
+
+
 class Reference:
     """The reference's own code (oracle/_ref/libsqref.so), used only to pin
     the oracle and to time the CPU baseline."""
@@ -159,6 +180,7 @@ class Reference:
         L.GTO_kinetic.restype = D
         L.GTO_nuclei.argtypes = [I, I, PG, PM]
         L.GTO_nuclei.restype = D
+        L.GTO_2JK_Matrix_Symm_Shell.argtypes = [I, PD, PG, PD, D, PD]
         self.libc = C.CDLL(None)
         self.libc.free.argtypes = [P]
 
@@ -182,6 +204,17 @@ class Reference:
         self.L.GTO_JK_Matrix_Quartet_Parallel(0, n, dptr(PA), dptr(PB), g, dptr(sch), fixed_cutoff,
                                               dptr(J), dptr(KA), dptr(KB), C.byref(opt))
         return J, KA, KB
+
+    def g_legacy(self, g, n, P, sch):
+        """The legacy RHF two-electron matrix of src/rhf.c:82-116:
+        G = 0.5 * GTO_2JK_Matrix_Symm_Shell(P) = J[P] - K[P]/2 for the RHF
+        density P = 2 sum C C^T, screened at the basis-function level at
+        1e-10 (src/matrix.c:854-1127)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        sch = np.ascontiguousarray(sch, dtype=np.float64)
+        G = np.zeros((n, n))
+        self.L.GTO_2JK_Matrix_Symm_Shell(n, dptr(P), g, dptr(sch), 1.0e-10, dptr(G))
+        return 0.5 * G
 
     def jk_reset(self, opt):
         z = np.zeros(1)

@@ -123,3 +123,61 @@ def test_density_file_round_trip(tmp_path):
     assert np.allclose(QA, PA, rtol=1e-8) and np.allclose(QB, PB, rtol=1e-8)
     L.sq_load_density(n, 1, path, dptr(QA), dptr(QB))  # restricted: beta = alpha
     assert np.array_equal(QA, QB)
+
+
+def _ref_check():
+    import pytest
+    from oracle import ReferenceCheck
+    try:
+        return ReferenceCheck()
+    except FileNotFoundError:
+        pytest.skip("oracle/_ref/libsqref_check.so not built")
+
+
+def _compare_loaded(Lr, opt, mol, bas, CA, CB, eA, eB, E):
+    """Reads a checkpoint through loader library Lr and compares it with the
+    data it was written from."""
+    n = bas.n
+    assert Lr.load_checkpoint_multiplicity(C.byref(opt)) == 2
+    m = Lr.load_checkpoint_molecule(C.byref(opt)).contents
+    assert m.nAtom == mol.s.nAtom and m.Q == mol.s.Q
+    assert all(m.Z[i] == mol.s.Z[i] and abs(m.x[i] - mol.s.x[i]) < 1e-7 and abs(m.z[i] - mol.s.z[i]) < 1e-7
+               for i in range(m.nAtom))
+    nb = C.c_int()
+    g = Lr.load_checkpoint_basis(C.byref(opt), C.byref(nb))
+    assert nb.value == n
+    for i in range(n):
+        a, b = g[i], bas.arr[i]
+        assert (a.l, a.m, a.n, a.nContract) == (b.l, b.m, b.n, b.nContract)
+        assert abs(a.x0 - b.x0) < 1e-7 and abs(a.y0 - b.y0) < 1e-7 and abs(a.z0 - b.z0) < 1e-7
+        for k in range(a.nContract):
+            assert abs(a.exp[k] / b.exp[k] - 1) < 1e-8 and abs(a.coef[k] - b.coef[k]) < 1e-8
+            assert abs(a.norm[k] / b.norm[k] - 1) < 1e-7
+    cA, cB, fA, fB = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+    E2 = Lr.load_checkpoint_orbital(n, dptr(cA), dptr(cB), dptr(fA), dptr(fB), C.byref(opt))
+    assert abs(E2 - E) < 1e-8
+    for x, y in ((cA, CA), (cB, CB), (fA, eA), (fB, eB)):
+        assert np.allclose(x, y, rtol=1e-7, atol=1e-12)
+
+
+def test_checkpoint_moves_between_programs(tmp_path):
+    """A checkpoint written here loads in the reference's own reader
+    (src/check.c:37-300, compiled into oracle/_ref), and one written by the
+    reference's save_checkpoint (src/check.c:432-570) loads here: molecule,
+    multiplicity, basis and orbitals of an L-shell (6-31G*) UHF doublet."""
+    L = lib()
+    _setup(L)
+    R = _ref_check().L
+    mol, bas = g2("nh3_q0m1", "631g_star", charge=1)
+    n = bas.n
+    S = _overlap(L, bas)
+    CA, CB = _orbitals(S, 11), _orbitals(S, 12)
+    eA = np.sort(np.random.default_rng(13).standard_normal(n))
+    eB = np.sort(np.random.default_rng(14).standard_normal(n))
+    E = -55.8734520312
+    for k, (writer, reader) in enumerate(((L, R), (R, L))):
+        opt = default_option(multiplicity=2, restricted=0)
+        opt.CheckFile = str(tmp_path / f"nh3_{k}.chk").encode()
+        writer.save_checkpoint(n, bas.ptr, mol.ptr, E, 2e-9, 3e-9, dptr(CA), dptr(CB), dptr(eA), dptr(eB),
+                               C.byref(opt))
+        _compare_loaded(reader, opt, mol, bas, CA, CB, eA, eB, E)

@@ -106,6 +106,24 @@ def test_schwarz_inequality(oracle):
         assert abs(L.or_contr_eri(bas.ptr, p, q, r, s)) <= S[p, q] * S[r, s] * (1 + 1e-12) + 1e-15
 
 
+def test_one_electron_vs_compiled_reference(oracle, reference):
+    """The oracle's contracted S, T, V (src/matrix.c:35-164 over
+    src/int.c:100-411) against the reference's own GTO_overlap,
+    GTO_kinetic and GTO_nuclei on an L-shell (6-31G*) and a d-shell basis."""
+    for name in ("nh3_q0m1", "h2o_q0m1"):
+        mol, bas = g2(name, "631g_star")
+        n = bas.n
+        S, T, V = oracle.one_electron(bas.ptr, n, mol.ptr)
+        L = reference.L
+        for i in range(n):
+            for j in range(i + 1):
+                s, t, v = (L.GTO_overlap(i, j, bas.ptr), L.GTO_kinetic(i, j, bas.ptr),
+                           L.GTO_nuclei(i, j, bas.ptr, mol.ptr))
+                assert abs(S[i, j] - s) <= 1e-13 * max(1.0, abs(s)), (i, j)
+                assert abs(T[i, j] - t) <= 1e-13 * max(1.0, abs(t)), (i, j)
+                assert abs(V[i, j] - v) <= 1e-12 * max(1.0, abs(v)), (i, j)
+
+
 def test_eri_vs_compiled_reference(oracle, reference):
     rng = np.random.default_rng(5)
     for _ in range(200):
