@@ -305,8 +305,11 @@ def main():
               for lq, wq, wp, _, _ in work]
     dom = int(np.argmax(cls_ms))
     dlq = work[dom][0]
-    dname = "k_class_wave<2, 2, 2, 2, 0, double>" if dlq == (2, 2, 2, 2) else \
-        "k_class<%d, %d, %d, %d, 0, double, %d>" % (dlq + (512 if workmodel.class_block(*dlq) == 512 else 256,))
+    if dlq == (2, 2, 2, 2):
+        dname = "k_class_wave<2, 2, 2, 2, 0, double>" if os.environ.get("SQJK_DD_KERNEL") == "wave" \
+            else "k_class_wd<2, 2, 2, 2, 0>"
+    else:
+        dname = "k_class<%d, %d, %d, %d, 0, double, %d>" % (dlq + (512 if workmodel.class_block(*dlq) == 512 else 256,))
     dach = cflops[dom] / (cls_ms[dom] / 1e3) / 1e12
     build_ach = sum(cflops) / (kernel_ms / 1e3) / 1e12
     traffic, traffic_src = profile_traffic(dname)

@@ -65,8 +65,9 @@ long long sqjk_footprint_bytes(int nBasis, const struct GTOBasis_t *gto, int uhf
  * (src/qmd.c:564-667, src/optimize.c:283-340): primitive pair data and the
  * Schwarz bounds are recomputed on the device, while the shells, pair
  * lists, device buffers and stream of the context are kept.  Returns
- * SQJK_EARG if the basis differs in anything but its centres. */
-int sqjk_update_geometry(sqjk_ctx *ctx, const struct GTOBasis_t *gto);
+ * SQJK_EARG if the basis (nBasis functions) differs in anything but its
+ * centres. */
+int sqjk_update_geometry(sqjk_ctx *ctx, int nBasis, const struct GTOBasis_t *gto);
 
 /* One J/K build on density PA (alpha) and PB (beta):
  *   JT += J[PA+PB],  KA -= K[PA],  KB -= K[PB]   (accumulating, as the

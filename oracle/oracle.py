@@ -141,6 +141,39 @@ class Oracle:
         return E, it.value
 
 
+def legacy_rhf(oracle, g, n, mol, nE, drag=0.25, scf_max=80):
+    """The reference's legacy RHF loop (src/rhf.c:286-548), restated on the
+    oracle's integrals: core-Hamiltonian guess (all eigenvectors of H C = S C
+    e), P = 2 C_occ C_occ^T (src/rhf.c:59-72), G = J[P] - K[P]/2 with the
+    Schwarz cut 1e-10, F = H + G, the energy E_NN + sum (H + F) P / 2 of the
+    new undamped density (src/rhf.c:124-144), then P = P_old + drag (P_new -
+    P_old), until |dE| <= 1e-6 and rms dP <= 1e-6 or scf_max iterations.
+    The eigenvectors come from scipy (LAPACK), the reference's own solver
+    family.  Returns (E, iterations)."""
+    import scipy.linalg as sl
+    nocc = nE // 2
+    S, T, V = oracle.one_electron(g, n, mol)
+    H = T + V
+    Enn = oracle.L.or_nuclei_coulomb(mol)
+    _, Cm = sl.eigh(H, S)
+    P = 2.0 * Cm[:, :nocc] @ Cm[:, :nocc].T
+    E = Enn + float(np.sum(H * P))
+    sch = oracle.schwarz(g, n)
+    ctx = oracle.jk_context(False, 0)
+    it = 0
+    while True:
+        it += 1
+        J, K, _ = oracle.jk(ctx, g, n, P / 2, P / 2, sch, 0.0, 1e-10, True)
+        F = H + J + K
+        _, Cm = sl.eigh(F, S)
+        Pn = 2.0 * Cm[:, :nocc] @ Cm[:, :nocc].T
+        avg = float(np.sqrt(np.sum((Pn - P) ** 2) / n / n))
+        Eold, E = E, Enn + 0.5 * float(np.sum((H + F) * Pn))
+        P = P + drag * (Pn - P)
+        if it >= scf_max or not (abs(E - Eold) > 1e-6 or avg > 1e-6):
+            return E, it
+
+
 class ReferenceCheck:
     """The reference's checkpoint reader/writer (src/check.c, compiled into
     oracle/_ref/libsqref_check.so).  Loaded with lazy binding: its one

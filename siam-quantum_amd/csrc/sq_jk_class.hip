@@ -5,6 +5,7 @@
 
 #include "sq_jk_classes.h"
 #include "sq_jk_wave.h"
+#include "sq_jk_wd.h"
 
 #if !defined(SQ_LA) || !defined(SQ_LB) || !defined(SQ_LC) || !defined(SQ_LD)
 #error "define SQ_LA SQ_LB SQ_LC SQ_LD"
@@ -50,6 +51,36 @@ int launch_wave(const ClassArgs& a, int grid, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
+// Wavefront-per-quartet kernel with precomputed Hermite tables (RHF/UHF
+// builds of one molecule): grid = at most `over` times the workgroups the
+// device holds at once, each of its four wavefronts striding over the
+// (tile, ket position) items.
+template <int MODE>
+int launch_wd(const ClassArgs& a, int grid, hipStream_t st) {
+  if constexpr (MODE == kSchwarz || !kWave) {
+    return -1;
+  } else {
+    static const long long resident = [] {
+      int dev = 0, ncu = 0, nb = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_class_wd<LA, LB, LC, LD, MODE>, 64 * kWDWaves, 0) !=
+              hipSuccess || nb < 1)
+        nb = 1;
+      return (long long)nb * ncu;
+    }();
+    const long long items = a.total * 64;
+    const long long need = (items + kWDWaves - 1) / kWDWaves;
+    const long long over = grid < 0 ? -grid : 4;
+    const long long cap = std::max(1LL, over * resident);
+    const long long g = grid > 0 ? grid : (need < cap ? need : cap);
+    hipLaunchKernelGGL((k_class_wd<LA, LB, LC, LD, MODE>), dim3((int)(g > 0 ? g : 1)), dim3(64 * kWDWaves), 0, st,
+                       a);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+
 // Workgroups of the per-thread kernel that the current device holds at once
 // (occupancy x CUs), computed once per kernel.
 template <int MODE, typename R>
@@ -81,6 +112,8 @@ long long resident_blocks() {
 template <int MODE, typename R>
 int launch_mode(const ClassArgs& a, int grid, hipStream_t st) {
   if constexpr (kWave) {
+    if constexpr (MODE != kSchwarz && std::is_same<R, double>::value)
+      if (a.herm) return launch_wd<MODE>(a, grid, st);
     const long long items = MODE == kSchwarz ? a.total : a.total * 64;
     const long long cap = grid < 0 ? (long long)(-grid) << 18 : (1 << 20);
     const long long g = grid > 0 ? grid : (items < cap ? items : cap);

@@ -147,6 +147,10 @@ struct ClassArgs {
   // ket pair class of the launch
   const InstDesc* inst;
   int ycls;
+  // Hermite coefficient tables of the class's primitive pairs (dd launches of
+  // one molecule; nullptr otherwise) and the primitive-pair index of entry 0
+  const double* herm;
+  int herm_base;
 };
 
 // The per-molecule part of the arguments, as one tile sees it: from the

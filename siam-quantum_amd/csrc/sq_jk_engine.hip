@@ -37,6 +37,7 @@
 
 #include "../../include/sq_jk.h"
 #include "sq_jk_classes.h"
+#include "sq_jk_wd.h"
 #include "sq_one_e.h"
 #include "sq_internal.h"
 
@@ -293,6 +294,7 @@ struct sqjk_ctx {
   std::vector<PairRec> pairs[kNPairClass];
   std::vector<double> sch_basis;  // n x n
   std::vector<double> sch_ref;    // nref x nref
+  std::vector<double> pexp, pcoef;  // primitive exponents and coefficients per sub-shell
   // device
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -307,8 +309,11 @@ struct sqjk_ctx {
   std::vector<Group> groups[kNPairClass];
   // work plan for the cut-offs (thr, thr64) of the last build
   struct Launch { int X, Y, fp32, seg0, tile0; long long ntile, quartets; };
-  bool plan_ok = false;
-  double plan_thr = 0.0, plan_thr64 = 0.0;
+  // plan_ok: the device plan (d_seg ... d_rhi) holds the host plan below,
+  // made for the cut-offs (plan_thr, plan_thr64); hplan_ok: the host plan
+  // was made for (hplan_thr, hplan_thr64)
+  bool plan_ok = false, hplan_ok = false;
+  double plan_thr = 0.0, plan_thr64 = 0.0, hplan_thr = 0.0, hplan_thr64 = 0.0;
   std::vector<Launch> launches;
   std::vector<Segment> h_seg;
   std::vector<Tile> h_tile;
@@ -330,6 +335,11 @@ struct sqjk_ctx {
   // pinned host staging of the densities (in) and of J, K (out)
   double *hp_A = nullptr, *hp_B = nullptr, *hp_J = nullptr, *hp_KA = nullptr, *hp_KB = nullptr;
   int* d_ref = nullptr;  // reference shells: first basis function, count
+  // Hermite coefficient tables of the dd-class primitive pairs (their
+  // primitive-pair indices are [herm_base, herm_base + herm_count))
+  double* d_herm = nullptr;
+  int herm_base = 0, herm_count = 0;
+  bool no_wd = false;  // SQJK_DD_KERNEL=wave: the cooperative (dd|dd) kernel instead
   sqjk_stats last{};
   // per class pair (X,Y) of the last build: contracted and primitive quartets
   long long cls_q[kNPairClass][kNPairClass] = {};
@@ -563,15 +573,52 @@ int upload_pairs(sqjk_ctx* c) {
   return SQJK_OK;
 }
 
-// Pair-indexed digestion buffers (alpha always, beta for UHF builds).
-int ensure_stage(sqjk_ctx* c, bool uhf) {
+// Hermite coefficient tables of the dd-class primitive pairs (geometry
+// only), for the wavefront-per-quartet (dd|dd) kernel.  The class's primitive
+// pairs are contiguous in the grouped primitive-major layout.
+int build_herm_tables(sqjk_ctx* c) {
+  constexpr int X = kNPairClass - 1;  // dd
+  using HL = HermTabLayout<2, 2>;
+  const auto& prs = c->pairs[X];
+  if (prs.empty()) return SQJK_OK;
+  int lo = INT32_MAX, hi = 0;
+  for (const PairRec& pr : prs) {
+    lo = std::min(lo, pr.pp_off);
+    hi = std::max(hi, pr.pp_off + (pr.npp - 1) * pr.pp_stride + 1);
+    if (pr.npp > 64) return fail(SQJK_EBASIS, "a dd shell pair has more than 64 primitive pairs");
+  }
+  const int count = hi - lo;
+  if (!c->d_herm || count > c->herm_count) {
+    if (c->d_herm) block_put(c->d_herm);
+    c->d_herm = nullptr;
+    if (int rc = dalloc(&c->d_herm, (size_t)count * HL::SIZE)) return rc;
+  }
+  c->herm_base = lo;
+  c->herm_count = count;
+  const long long work = (long long)prs.size() * 64;
+  hipLaunchKernelGGL((k_herm_table<2, 2>), dim3(grid_for(work, 256)), dim3(256), 0, c->st, c->d_pairs[X],
+                     (int)prs.size(), c->d_pp, c->d_sub, lo, c->d_herm);
+  HIPCHK(hipGetLastError());
+  return SQJK_OK;
+}
+
+// Lengths of the pair-indexed digestion buffers: Jp [NAB][np] and the
+// exchange staging S [n][NC+ND][np] of every pair class.
+void stage_lengths(sqjk_ctx* c) {
   for (int X = 0; X < kNPairClass; ++X) {
     const size_t np = c->pairs[X].size();
     const int nab = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]);
     const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
     c->jp_len[X] = np * nab;
     c->s_len[X] = (size_t)c->n * ncomp * np;
-    if (np == 0) continue;
+  }
+}
+
+// Pair-indexed digestion buffers (alpha always, beta for UHF builds).
+int ensure_stage(sqjk_ctx* c, bool uhf) {
+  stage_lengths(c);
+  for (int X = 0; X < kNPairClass; ++X) {
+    if (c->pairs[X].empty()) continue;
     int rc;
     if (!c->d_Jp[X] && (rc = dalloc(&c->d_Jp[X], c->jp_len[X]))) return rc;
     if (!c->d_SA[X] && (rc = dalloc(&c->d_SA[X], c->s_len[X]))) return rc;
@@ -648,7 +695,7 @@ void finish_pairs(sqjk_ctx* c) {
     }
   }
   c->pp.swap(ppT);
-  c->plan_ok = false;
+  c->plan_ok = c->hplan_ok = false;
 }
 
 // Rows and tiles of every launch for the cut-offs (thr, thr64): quartets with
@@ -719,6 +766,10 @@ void init_tuning(sqjk_ctx* c) {
 }
 
 void build_plan(sqjk_ctx* c, double thr, double thr64) {
+  c->plan_ok = false;  // the device plan no longer holds the host plan
+  c->hplan_ok = true;
+  c->hplan_thr = thr;
+  c->hplan_thr64 = thr64;
   c->launches.clear();
   c->h_seg.clear();
   c->h_tile.clear();
@@ -829,9 +880,14 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
       }
 }
 
+// Host plan for the cut-offs (thr, thr64), made only when they changed.
+void host_plan(sqjk_ctx* c, double thr, double thr64) {
+  if (!(c->hplan_ok && c->hplan_thr == thr && c->hplan_thr64 == thr64)) build_plan(c, thr, thr64);
+}
+
 int make_plan(sqjk_ctx* c, double thr, double thr64) {
   if (c->plan_ok && c->plan_thr == thr && c->plan_thr64 == thr64) return SQJK_OK;
-  build_plan(c, thr, thr64);
+  host_plan(c, thr, thr64);
   if (c->h_rbra.size() > (size_t)INT32_MAX) return fail(SQJK_ENOMEM, "work plan too large");
   auto grow = [&](auto** d, size_t& cap, size_t need) -> int {
     if (cap >= need && *d) return SQJK_OK;
@@ -884,10 +940,13 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   c->device = device;
   c->n = nBasis;
   init_tuning(c);
+  if (const char* e = std::getenv("SQJK_DD_KERNEL")) c->no_wd = std::string(e) == "wave";
   std::vector<double> pexp, pcoef;
   int rc = plan_shells(c, nBasis, gto, pexp, pcoef);
   if (rc) { delete c; return rc; }
   plan_pairs(c, pexp, pcoef);
+  c->pexp = pexp;
+  c->pcoef = pcoef;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     sqjk_destroy(c);
@@ -908,6 +967,7 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
     sqjk_destroy(c);
     return fail(SQJK_EHIP, "primitive pair upload failed");
   }
+  if ((rc = build_herm_tables(c))) { sqjk_destroy(c); return rc; }
   if (hipStreamSynchronize(c->st) != hipSuccess) { sqjk_destroy(c); return fail(SQJK_EHIP, "sync failed"); }
   *out = c;
   return SQJK_OK;
@@ -929,12 +989,14 @@ long long sqjk_footprint_bytes(int nBasis, const GTOBasis_t* gto, int uhf, int w
     const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
     b += np * sizeof(PairRec) + np * nab * 8 + (uhf ? 2 : 1) * (long long)nBasis * ncomp * np * 8;
   }
+  for (const PairRec& pr : t.pairs[kNPairClass - 1]) b += (long long)pr.npp * HermTabLayout<2, 2>::SIZE * 8;
   if (with_scf) b += 38 * nn * 8;  // SCF matrices, DIIS history, eigen-solver work
   return b + b / 8;                 // work plan and allocator slack
 }
 
-int sqjk_update_geometry(sqjk_ctx* c, const GTOBasis_t* gto) {
+int sqjk_update_geometry(sqjk_ctx* c, int nBasis, const GTOBasis_t* gto) {
   if (!c || !gto) return fail(SQJK_EARG, "sqjk_update_geometry: null argument");
+  if (nBasis != c->n) return fail(SQJK_EARG, "sqjk_update_geometry: the basis has a different number of functions");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->st));
   // the new basis must have the same shells, primitives and coefficients;
@@ -950,6 +1012,7 @@ int sqjk_update_geometry(sqjk_ctx* c, const GTOBasis_t* gto) {
     same = a.l == b.l && a.nprim == b.nprim && a.bf_off == b.bf_off && a.parent == b.parent;
   }
   for (int f = 0; same && f < c->n; ++f) same = t.bscale[f] == c->bscale[f];
+  same = same && pexp == c->pexp && pcoef == c->pcoef;
   if (!same) return fail(SQJK_EARG, "sqjk_update_geometry: the basis differs in more than its centres");
   // new centres: primitive pairs (pair order), Schwarz on the device, then
   // groups and the primitive-major layout as at creation; every buffer and
@@ -969,8 +1032,9 @@ int sqjk_update_geometry(sqjk_ctx* c, const GTOBasis_t* gto) {
       HIPCHK(hipMemcpyAsync(c->d_pairs[X], c->pairs[X].data(), c->pairs[X].size() * sizeof(PairRec),
                             hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_pp, c->pp.data(), c->pp.size() * sizeof(PrimPair), hipMemcpyHostToDevice, c->st));
+  if ((rc = build_herm_tables(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
-  c->plan_ok = false;
+  c->plan_ok = c->hplan_ok = false;
   c->built = false;
   return SQJK_OK;
 }
@@ -1055,13 +1119,80 @@ int sqjk_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, 
 
 namespace {
 
+// Device buffers of one build: the work plan and the pair-indexed digestion
+// buffers of every class (exchange staging per spin).
+struct BuildBufs {
+  Segment* seg = nullptr;
+  Tile* tile = nullptr;
+  int *rbra = nullptr, *rlo = nullptr, *rhi = nullptr;
+  double* Jp[kNPairClass] = {};
+  double* SA[kNPairClass] = {};
+  double* SB[kNPairClass] = {};
+};
+
+// The context's own buffers (make_plan and ensure_stage have run).
+BuildBufs context_bufs(const sqjk_ctx* c) {
+  BuildBufs B;
+  B.seg = c->d_seg; B.tile = c->d_tile; B.rbra = c->d_rbra; B.rlo = c->d_rlo; B.rhi = c->d_rhi;
+  for (int X = 0; X < kNPairClass; ++X) { B.Jp[X] = c->d_Jp[X]; B.SA[X] = c->d_SA[X]; B.SB[X] = c->d_SB[X]; }
+  return B;
+}
+
+// The same buffers carved out of a caller-provided region of `cap` bytes
+// (256-byte aligned pieces), with the host plan for the cut-offs uploaded
+// into it.  Nothing is allocated on the device.
+int carve_bufs(sqjk_ctx* c, double thr, double thr64, bool uhf, char* region, size_t cap, BuildBufs* B) {
+  host_plan(c, thr, thr64);
+  stage_lengths(c);
+  const size_t nrow = c->h_rbra.size();
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t o_seg = take(c->h_seg.size() * sizeof(Segment)), o_tile = take(c->h_tile.size() * sizeof(Tile));
+  const size_t o_rbra = take(nrow * sizeof(int)), o_rlo = take(nrow * sizeof(int)), o_rhi = take(nrow * sizeof(int));
+  size_t o_jp[kNPairClass], o_sa[kNPairClass], o_sb[kNPairClass];
+  for (int X = 0; X < kNPairClass; ++X) {
+    o_jp[X] = take(c->jp_len[X] * sizeof(double));
+    o_sa[X] = take(c->s_len[X] * sizeof(double));
+    o_sb[X] = uhf ? take(c->s_len[X] * sizeof(double)) : 0;
+  }
+  if (off > cap)
+    return fail(SQJK_ENOMEM, "sqjk_build_device_noalloc: this build needs " + std::to_string(off >> 20) +
+                             " MiB of scratch, above the " + std::to_string(cap >> 20) + " MiB the library holds");
+  B->seg = reinterpret_cast<Segment*>(region + o_seg);
+  B->tile = reinterpret_cast<Tile*>(region + o_tile);
+  B->rbra = reinterpret_cast<int*>(region + o_rbra);
+  B->rlo = reinterpret_cast<int*>(region + o_rlo);
+  B->rhi = reinterpret_cast<int*>(region + o_rhi);
+  for (int X = 0; X < kNPairClass; ++X) {
+    B->Jp[X] = reinterpret_cast<double*>(region + o_jp[X]);
+    B->SA[X] = reinterpret_cast<double*>(region + o_sa[X]);
+    B->SB[X] = uhf ? reinterpret_cast<double*>(region + o_sb[X]) : nullptr;
+  }
+  if (!c->h_tile.empty()) {
+    HIPCHK(hipMemcpyAsync(B->seg, c->h_seg.data(), c->h_seg.size() * sizeof(Segment), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(B->tile, c->h_tile.data(), c->h_tile.size() * sizeof(Tile), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(B->rbra, c->h_rbra.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(B->rlo, c->h_rlo.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(B->rhi, c->h_rhi.data(), nrow * sizeof(int), hipMemcpyHostToDevice, c->st));
+  }
+  return SQJK_OK;
+}
+
 // The device part of a build.  The densities are in c->d_DA, c->d_DB; the
 // results J and -K (full, symmetric) are written to oJ, oKA and, for UHF,
-// oKB.  Everything is queued on the context's stream; nothing waits.
+// oKB.  Everything is queued on the context's stream; nothing waits.  The
+// work plan and digestion buffers are the context's own (allocated on first
+// use) unless `region` (cap bytes of device memory) is given: then they are
+// carved out of it, nothing is allocated, and per-class profiling is off.
 int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, double* oJ, double* oKA,
-                    double* oKB) {
+                    double* oKB, char* region = nullptr, size_t cap = 0) {
   const int n = c->n, nr = c->nref;
   const size_t nn = (size_t)n * n;
+  if ((size_t)n * sizeof(double) > 64 * 1024) return fail(SQJK_EARG, "nBasis above 8192 is not supported");
   hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
                      c->d_DJ, nn);
   hipLaunchKernelGGL(k_pair_max, dim3(grid_for((long long)nr * nr, 256)), dim3(256), 0, c->st, c->d_DJ, c->d_DA,
@@ -1071,10 +1202,16 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
   // work plan (rows/segments), rebuilt only when the cut-offs change
   const double thr = std::max(fixedCutoff, scfCutoff);
   const double thr64 = std::max(thr, c->mixed_thr);
-  int rc = make_plan(c, thr, thr64);
-  if (rc) return rc;
-  if ((size_t)n * sizeof(double) > 64 * 1024) return fail(SQJK_EARG, "nBasis above 8192 is not supported");
-  if ((rc = ensure_stage(c, uhf))) return rc;
+  BuildBufs B;
+  int rc;
+  if (region) {
+    if ((rc = carve_bufs(c, thr, thr64, uhf != 0, region, cap, &B))) return rc;
+  } else {
+    if ((rc = make_plan(c, thr, thr64))) return rc;
+    if ((rc = ensure_stage(c, uhf))) return rc;
+    B = context_bufs(c);
+  }
+  const bool prof = c->profile && !region;
   HIPCHK(hipEventRecord(c->ev0, c->st));
   const int fill_grid = grid_for((long long)nn, 256);
   hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, nn, 0.0);
@@ -1082,44 +1219,48 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
   if (uhf) hipLaunchKernelGGL(k_fill, dim3(fill_grid), dim3(256), 0, c->st, c->d_AKB, nn, 0.0);
   for (int X = 0; X < kNPairClass; ++X) {
     if (c->pairs[X].empty()) continue;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)c->jp_len[X], 256)), dim3(256), 0, c->st, c->d_Jp[X],
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)c->jp_len[X], 256)), dim3(256), 0, c->st, B.Jp[X],
                        c->jp_len[X], 0.0);
     const int sg = std::max(1, (int)std::min<size_t>((c->s_len[X] + 255) / 256, 16384));
-    hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SA[X], c->s_len[X], 0.0);
-    if (uhf) hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SB[X], c->s_len[X], 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, B.SA[X], c->s_len[X], 0.0);
+    if (uhf) hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, B.SB[X], c->s_len[X], 0.0);
   }
   HIPCHK(hipGetLastError());
   int launches = 0;
-  if (c->profile && c->prof_ev.size() < 2 * c->launches.size()) {
+  if (prof && c->prof_ev.size() < 2 * c->launches.size()) {
     const size_t have = c->prof_ev.size();
     c->prof_ev.resize(2 * c->launches.size());
     for (size_t e = have; e < c->prof_ev.size(); ++e) HIPCHK(hipEventCreate(&c->prof_ev[e]));
   }
   for (size_t li = 0; li < c->launches.size(); ++li) {
     const sqjk_ctx::Launch& L = c->launches[li];
-    if (c->profile) HIPCHK(hipEventRecord(c->prof_ev[2 * li], c->st));
+    if (prof) HIPCHK(hipEventRecord(c->prof_ev[2 * li], c->st));
     ClassArgs a = base_args(c);
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
-    a.seg = c->d_seg;
-    a.tile = c->d_tile + L.tile0;
-    a.rbra = c->d_rbra;
-    a.rlo = c->d_rlo;
-    a.rhi = c->d_rhi;
+    a.seg = B.seg;
+    a.tile = B.tile + L.tile0;
+    a.rbra = B.rbra;
+    a.rlo = B.rlo;
+    a.rhi = B.rhi;
     a.same = (L.X == L.Y);
     a.total = L.ntile;
     a.fixed_cut = fixedCutoff;
-    a.JpB = c->d_Jp[L.X];
-    a.JpK = c->d_Jp[L.Y];
-    a.SA = c->d_SA[L.Y];
-    a.SB = c->d_SB[L.Y];
+    a.JpB = B.Jp[L.X];
+    a.JpK = B.Jp[L.Y];
+    a.SA = B.SA[L.Y];
+    a.SB = B.SB[L.Y];
     a.npb = (int)c->pairs[L.X].size();
     a.npk = (int)c->pairs[L.Y].size();
+    if (L.X == kNPairClass - 1 && L.Y == kNPairClass - 1 && c->d_herm && !c->no_wd) {
+      a.herm = c->d_herm;
+      a.herm_base = c->herm_base;
+    }
     ClassLauncher f = launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
     const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, -c->grid_over[L.X][L.Y], c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
-    if (c->profile) HIPCHK(hipEventRecord(c->prof_ev[2 * li + 1], c->st));
+    if (prof) HIPCHK(hipEventRecord(c->prof_ev[2 * li + 1], c->st));
   }
   // fold the pair-indexed buffers into the half-accumulators
   KStage RA{}, RB{};
@@ -1127,10 +1268,10 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
     const int np = (int)c->pairs[X].size();
     if (np == 0) continue;
     const int nb = sqmd::ncart(kPairL[X][1]), nab = sqmd::ncart(kPairL[X][0]) * nb;
-    hipLaunchKernelGGL(k_reduce_j, dim3(grid_for((long long)np * nab, 256)), dim3(256), 0, c->st, c->d_Jp[X],
+    hipLaunchKernelGGL(k_reduce_j, dim3(grid_for((long long)np * nab, 256)), dim3(256), 0, c->st, B.Jp[X],
                        c->d_pairs[X], c->d_sub, np, nb, nab, c->d_AJ, n);
-    RA.S[X] = c->d_SA[X];
-    RB.S[X] = c->d_SB[X];
+    RA.S[X] = B.SA[X];
+    RB.S[X] = B.SB[X];
     RA.pr[X] = RB.pr[X] = c->d_pairs[X];
     RA.np[X] = RB.np[X] = np;
     RA.nc[X] = RB.nc[X] = sqmd::ncart(kPairL[X][0]);
@@ -1332,6 +1473,9 @@ int sqjk_class_work(sqjk_ctx* c, long long* quartets, long long* prim_quartets, 
                     long long* prim_quartets_fp32) {
   if (!c) return fail(SQJK_EARG, "sqjk_class_work: null context");
   if (!c->built) return fail(SQJK_EARG, "sqjk_class_work: no build has run on this context");
+  if (!c->plan_ok)
+    return fail(SQJK_EARG, "sqjk_class_work: the last build's work plan is not held by the context "
+                           "(a batched or allocation-free build ran since)");
   HIPCHK(hipSetDevice(c->device));
   const size_t nl = c->launches.size();
   unsigned long long* d_cnt = nullptr;
@@ -1394,7 +1538,7 @@ int sqjk_set_tuning(sqjk_ctx* c, const int* rows, const int* split_budget, const
         if (grid_over) c->grid_over[X][Y] = grid_over[X * kNPairClass + Y];
       }
   }
-  c->plan_ok = false;
+  c->plan_ok = c->hplan_ok = false;
   return SQJK_OK;
 }
 
@@ -1441,6 +1585,7 @@ void sqjk_destroy(sqjk_ctx* c) {
   for (void* p : ptrs)
     block_put(p);
   block_put(c->d_ref);
+  block_put(c->d_herm);
   double* pins[] = {c->hp_A, c->hp_B, c->hp_J, c->hp_KA, c->hp_KB};
   for (double* h : pins)
     block_put(h);

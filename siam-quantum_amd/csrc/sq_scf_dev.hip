@@ -336,7 +336,7 @@ int sqscf_set_system(sqscf_ctx* c, int n, const struct GTOBasis_t* gto, const do
   if (!c || n <= 0 || !gto || !H || !S) return scf_fail(SQJK_EARG, "sqscf_set_system: invalid arguments");
   SCFCHK(hipSetDevice(c->device));
   int rc;
-  if (c->jk && c->n == n && sqjk_update_geometry(c->jk, gto) == SQJK_OK) {
+  if (c->jk && c->n == n && sqjk_update_geometry(c->jk, n, gto) == SQJK_OK) {
     // same basis at a new geometry: the J/K context is kept
   } else {
     if (c->jk) sqjk_destroy(c->jk);

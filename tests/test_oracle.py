@@ -221,3 +221,22 @@ def test_scf_reference_settings_water(oracle):
     E1, _ = oracle.uhf(bas.ptr, bas.n, mol.ptr, 5, 5, scf_conv=1e-8, three_step=True, use_mpole=True, max_mem=250)
     E2, _ = oracle.uhf(bas.ptr, bas.n, mol.ptr, 5, 5, scf_conv=1e-10, three_step=False, use_mpole=False, max_mem=0)
     assert abs(E1 - E2) < 5e-8
+
+
+def test_legacy_rhf_restatement(oracle, reference):
+    """oracle.legacy_rhf (src/rhf.c:286-548 restated): its two-electron
+    matrix J[P] - K[P]/2 is the reference's own getGMatrix (0.5 *
+    GTO_2JK_Matrix_Symm_Shell, src/rhf.c:82-116) to 1e-9 (the reference cuts
+    single function pairs at 1e-10, the oracle whole shells), and the damped
+    loop ends within its own stopping rule of the converged RHF energy."""
+    from oracle import legacy_rhf
+    mol, bas = g2("h2o_q0m1", "631g_star")
+    n = bas.n
+    P = random_density(n, 7, 0.05)
+    sch = reference.schwarz(bas.ptr, n)
+    J, K, _ = oracle.jk(oracle.jk_context(False, 0), bas.ptr, n, P / 2, P / 2, sch, 0.0, 1e-10, True)
+    assert np.abs(J + K - reference.g_legacy(bas.ptr, n, P, sch)).max() < 1e-9
+    E, it = legacy_rhf(oracle, bas.ptr, n, mol.ptr, mol.n_electrons, scf_max=400)
+    Eo, _ = oracle.uhf(bas.ptr, n, mol.ptr, 5, 5, restricted=True, scf_conv=1e-10, three_step=False,
+                       use_mpole=False, max_mem=0)
+    assert it < 400 and abs(E - Eo) < 1e-5

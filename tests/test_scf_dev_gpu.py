@@ -73,15 +73,22 @@ def test_geometry_steps_reuse_context():
     print(f"step 0: E {r0['energy']:.10f} ({r0['iterations']} it)  uhf() {E0:.10f}")
     assert abs(r0["energy"] - E0) < 1e-8
     for step in (1, 2):
-        mol = _displaced(mol0, 100 + step)
+        mol = _displaced(mol0, 100 + step, sigma_ang=0.02)
         bas = inputs.make_basis(mol, "631g_star")
         scf.set_system(bas, mol)
         scf.guess_from_orbitals(ne // 2, ne // 2)
         r = scf.run(ne // 2, ne // 2, conv=1e-10, three_step=False)
         Ef = _uhf(mol, bas, ne // 2, ne // 2, SCFConv=1e-10, SCFAccuracy=1)
-        print(f"step {step}: E {r['energy']:.10f} ({r['iterations']} it)  fresh uhf() {Ef:.10f}")
-        assert r["converged"] and abs(r["energy"] - Ef) < 1e-8
-        assert r["iterations"] < r0["iterations"]
+        # the same geometry from the DIAG guess on a fresh context
+        fresh = DeviceSCF()
+        _, S1 = fresh.set_system(bas, mol)
+        fresh.diag_guess(S1, ne // 2, ne // 2)
+        rf = fresh.run(ne // 2, ne // 2, conv=1e-10, three_step=False)
+        fresh.close()
+        print(f"step {step}: E {r['energy']:.10f} ({r['iterations']} it)  fresh uhf() {Ef:.10f}  "
+              f"fresh DIAG start {rf['energy']:.10f} ({rf['iterations']} it)")
+        assert r["converged"] and abs(r["energy"] - Ef) < 1e-8 and abs(rf["energy"] - Ef) < 1e-8
+        assert r["iterations"] < rf["iterations"]
     scf.close()
 
 
@@ -94,8 +101,8 @@ def test_update_geometry_matches_fresh_context():
     eng = JKEngine(bas0.ptr, n)
     eng.build(P, P, 1e-11, 1e-11, True)
     L = lib()
-    L.sqjk_update_geometry.argtypes = [C.c_void_p, C.POINTER(GTOBasis)]
-    assert L.sqjk_update_geometry(eng._h, bas1.ptr) == 0
+    L.sqjk_update_geometry.argtypes = [C.c_void_p, C.c_int, C.POINTER(GTOBasis)]
+    assert L.sqjk_update_geometry(eng._h, bas1.n, bas1.ptr) == 0
     J1, K1, _ = eng.build(P, P, 1e-11, 1e-11, True)
     s1 = eng.schwarz()
     eng.close()
@@ -106,8 +113,12 @@ def test_update_geometry_matches_fresh_context():
     d = max(np.abs(J1 - J2).max(), np.abs(K1 - K2).max(), np.abs(s1 - s2).max())
     print(f"moved vs fresh context: max diff {d:.2e}")
     assert d < 1e-12
-    # a different basis is refused
+    # a different basis is refused: another size, or the same functions
+    # with one exponent changed
     mol2, bas2 = g2("c2h4_q0m1", "sto3g")
     eng = JKEngine(bas0.ptr, n)
-    assert L.sqjk_update_geometry(eng._h, bas2.ptr) != 0
+    assert L.sqjk_update_geometry(eng._h, bas2.n, bas2.ptr) != 0
+    bas3 = inputs.make_basis(mol1, "631g_star")
+    bas3.arr[0].exp[0] *= 1.01
+    assert L.sqjk_update_geometry(eng._h, bas3.n, bas3.ptr) != 0
     eng.close()
