@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--scf-solves", type=int, default=100, help="warm SCF solves per precision (0: skip)")
     ap.add_argument("--scf-workload", default="w16", choices=["w16", "w32"])
     ap.add_argument("--no-extras", action="store_true", help="only the headline C60 build line")
+    ap.add_argument("--no-tuning", action="store_true", help="default work-plan tables, not the tuned ones")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -296,6 +297,13 @@ def main():
     print(f"bench: C60 E = {E_scf:.8f} Eh in {t_scf:.1f} s; timing builds", file=sys.stderr, flush=True)
     eng = JKEngine(bas.ptr, n)
     eng.set_profile(True)
+    tuning = None
+    from sqamd import tune
+    tpath = os.path.join(tune.DATA, "c60.json")
+    if os.path.exists(tpath) and not a.no_tuning:
+        t = tune.load(tpath)
+        eng.set_tuning(t["rows"], t["split"], t["grid"])
+        tuning = os.path.relpath(tpath, ROOT)
     wall, kernel_ms, cls_ms = timed_builds(eng, P, a.cutoff, a.steps, a.warmup)
     st = eng.stats()
     plan = eng.class_stats()
@@ -332,7 +340,7 @@ def main():
                    "quartets_surviving": int(sum(w[1] for w in work)),
                    "prim_quartets_surviving": int(sum(w[2] for w in work)),
                    "integrals_surviving": int(sum(wq * workmodel.nint(*lq) for lq, wq, _, _, _ in work)),
-                   "scf_energy": E_scf, "scf_seconds": t_scf},
+                   "scf_energy": E_scf, "scf_seconds": t_scf, "tuning": tuning},
         "kernel_ms": kernel_ms,
         "classes": {class_label(lq): {"quartets_plan": plan[k][1], "quartets": wq, "prim_quartets": wp,
                                       "gflop": cflops[k] / 1e9, "kernel_ms": float(cls_ms[k]),

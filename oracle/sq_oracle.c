@@ -20,7 +20,21 @@
  *   J/K digestion       src/matrix.c:2934-3396 (six PQIJ cases)
  *   J/K driver          src/matrix.c:3703-3986
  *   SCF                 src/uhf.c:48-315, 1171-1783; src/conv.c:606-844
- * Parity status of each part is stated in oracle/README.md.
+ * How each part is pinned (tests/test_oracle.py unless named):
+ *   Boys                mpmath quadrature; recursion invariant; bit-identical
+ *                       to the compiled reference (oracle/_ref)
+ *   primitive ERI       closed-form (ss|ss); Szabo-Ostlund H2/STO-3G values;
+ *                       permutation symmetry; Schwarz inequality; 200 random
+ *                       primitives up to d against the compiled reference
+ *   1-e integrals       S, T, V against the compiled reference's GTO_overlap,
+ *                       GTO_kinetic, GTO_nuclei
+ *   J/K + screening     brute-force dense J/K; the compiled reference's
+ *                       GTO_JK_Matrix_Quartet_Parallel (exact, screened, with
+ *                       monopole substitution)
+ *   readers             tests/test_host_c.py
+ *   SCF                 GAMESS and the reference's own log energies
+ * Parity unpinned: eigenvector phases and the DIIS path of the SCF (only
+ * converged energies fix them).
  */
 #include <math.h>
 #include <stdio.h>

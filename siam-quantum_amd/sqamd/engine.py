@@ -82,6 +82,17 @@ class JKEngine:
                                 int(bool(restricted)), dptr(J), dptr(KA), dptr(KB)), "sqjk_build")
         return J, KA, KB
 
+    def set_tuning(self, rows=None, split=None, grid=None):
+        """Work-plan tuning tables (6 x 6 ints each, see sqjk_set_tuning);
+        all None restores the defaults."""
+        L = lib()
+        P32 = C.POINTER(C.c_int)
+        L.sqjk_set_tuning.argtypes = [C.c_void_p, P32, P32, P32]
+        arr = [None if t is None else np.ascontiguousarray(np.asarray(t).reshape(36), dtype=np.int32)
+               for t in (rows, split, grid)]
+        _check(L.sqjk_set_tuning(self._h, *[None if a is None else a.ctypes.data_as(P32) for a in arr]),
+               "sqjk_set_tuning")
+
     def set_mixed(self, threshold):
         """Quartets with Schwarz product below `threshold` run in FP32 (0: all FP64)."""
         _check(lib().sqjk_set_mixed(self._h, float(threshold)), "sqjk_set_mixed")
