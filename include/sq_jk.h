@@ -90,6 +90,24 @@ int sqjk_build(sqjk_ctx *ctx, const double *PA, const double *PB, double fixedCu
  * outputs untouched until then.  No counterpart in the reference. */
 int sqjk_build_device(sqjk_ctx *ctx, const double *dPA, const double *dPB, double fixedCutoff,
                       double scfCutoff, int restricted, double *dJ, double *dKA, double *dKB);
+/* Allocation-free variant of sqjk_build_device, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same inputs, outputs, screening
+ * and results.  The work plan and the digestion buffers live in a scratch
+ * array of SQJK_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it
+ * runs on for the life of the process; the context's own buffers (made by
+ * sqjk_create) hold the densities and accumulators.  The call makes no
+ * device or pinned-host allocation (it rebuilds the context's host work
+ * plan when the cut-offs change).  A build that needs more scratch than
+ * that returns SQJK_ENOMEM with the size in sqjk_errmsg.  Every call on one
+ * GPU shares that GPU's scratch, so calls that overlap on one GPU (several
+ * contexts' streams, several host threads) race: finish one (sqjk_sync)
+ * before the next starts.  Per-class profiling is off for these builds.
+ * No counterpart in the reference; it exists for the callers above. */
+#define SQJK_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sqjk_build_device_noalloc(sqjk_ctx *ctx, const double *dPA, const double *dPB, double fixedCutoff,
+                              double scfCutoff, int restricted, double *dJ, double *dKA, double *dKB);
+
 /* Waits for the context's queued work (and reads its timing). */
 int sqjk_sync(sqjk_ctx *ctx);
 

@@ -1395,6 +1395,57 @@ int sqjk_build_device(sqjk_ctx* c, const double* dPA, const double* dPB, double 
   return SQJK_OK;
 }
 
+}  // extern "C"
+
+// Device memory of the allocation-free build (sqjk_build_device_noalloc):
+// its work plan and digestion buffers are carved out of this array.  Every
+// GPU the library runs on holds its own copy, at its own address, for the
+// life of the process.  In host code `scratch` names a host-side stand-in,
+// so kernels are given the device address from hipGetSymbolAddress, kept per
+// device below.
+constexpr size_t kScratchBytes = SQJK_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (the device must be current).
+int scratch_region(int device, char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SQJK_EARG, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SQJK_EHIP, std::string("scratch address lookup failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return SQJK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sqjk_build_device_noalloc(sqjk_ctx* c, const double* dPA, const double* dPB, double fixedCutoff,
+                              double scfCutoff, int restricted, double* dJ, double* dKA, double* dKB) {
+  if (!c || !dPA || !dPB || !dJ || !dKA || !dKB) return fail(SQJK_EARG, "sqjk_build_device_noalloc: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  char* region = nullptr;
+  int rc = scratch_region(c->device, &region);
+  if (rc) return rc;
+  const size_t nn = (size_t)c->n * c->n;
+  const int uhf = restricted ? 0 : 1;
+  HIPCHK(hipMemcpyAsync(c->d_DA, dPA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_DB, dPB, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  if ((rc = build_on_device(c, fixedCutoff, scfCutoff, uhf, dJ, dKA, dKB, region, kScratchBytes))) return rc;
+  if (!uhf && dKB != dKA) HIPCHK(hipMemcpyAsync(dKB, dKA, nn * 8, hipMemcpyDeviceToDevice, c->st));
+  c->last.total_ms = 0.0;
+  return SQJK_OK;
+}
+
 int sqjk_sync(sqjk_ctx* c) {
   if (!c) return fail(SQJK_EARG, "sqjk_sync: null context");
   HIPCHK(hipSetDevice(c->device));
