@@ -151,6 +151,9 @@ struct ClassArgs {
   // one molecule; nullptr otherwise) and the primitive-pair index of entry 0
   const double* herm;
   int herm_base;
+  // exchange staging of one molecule held in ket-pair chunks: SA/SB hold the
+  // ket pairs [s_k0, s_k0 + s_np) of the class (s_np = 0: the whole class)
+  int s_k0, s_np;
 };
 
 // The per-molecule part of the arguments, as one tile sees it: from the
@@ -176,7 +179,7 @@ __host__ __device__ __forceinline__ View make_view(const ClassArgs& A, int inst)
   if (A.inst == nullptr) {
     v.DJ = A.DJ; v.DA = A.DA; v.DB = A.DB; v.PM = A.PM;
     v.AJ = A.AJ; v.AKA = A.AKA; v.AKB = A.AKB; v.SA = A.SA; v.SB = A.SB;
-    v.n = A.n; v.nref = A.nref; v.npk = A.npk; v.k0 = 0; v.fixed_cut = A.fixed_cut;
+    v.n = A.n; v.nref = A.nref; v.npk = A.s_np > 0 ? A.s_np : A.npk; v.k0 = A.s_k0; v.fixed_cut = A.fixed_cut;
   } else {
     const InstDesc* d = A.inst + inst;
     v.DJ = uload(&d->DJ); v.DA = uload(&d->DA); v.DB = uload(&d->DB); v.PM = uload(&d->PM);

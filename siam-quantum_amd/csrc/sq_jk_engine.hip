@@ -169,10 +169,12 @@ __global__ void k_reduce_j(const double* __restrict__ Jp, const PairRec* __restr
 // Exchange staging of the per-thread kernels, one ket class per slot:
 // S[x][comp][j] belongs to row x of A_K at column bf(c of pair j) + comp
 // (comp < nc) or bf(d of pair j) + comp - nc.
+// Exchange staging of a fold: per ket class Y, S[Y] holds [n][ncomp][np]
+// for the ket pairs [p0, p0 + np) of the class, of which the first cnt exist.
 struct KStage {
   const double* S[6];
   const PairRec* pr[6];
-  int np[6], nc[6], ncomp[6];
+  int np[6], nc[6], ncomp[6], p0[6], cnt[6];
 };
 
 // One workgroup per row x of A_K: the row is summed in LDS, then added to A_K.
@@ -187,9 +189,9 @@ __global__ void __launch_bounds__(kReduceKBlock) k_reduce_k(KStage R, const SubS
     __syncthreads();
     for (int Y = 0; Y < 6; ++Y) {
       if (!R.S[Y]) continue;
-      const int np = R.np[Y], nc = R.nc[Y], ncomp = R.ncomp[Y];
+      const int np = R.np[Y], nc = R.nc[Y], ncomp = R.ncomp[Y], cnt = R.cnt[Y];
       const double* Sx = R.S[Y] + (size_t)x * ncomp * np;
-      const PairRec* prY = R.pr[Y];
+      const PairRec* prY = R.pr[Y] + R.p0[Y];
       const long long tot = (long long)ncomp * np;
       const long long bs = blockDim.x;
       for (long long e0 = threadIdx.x; e0 < tot; e0 += 4 * bs) {
@@ -201,6 +203,7 @@ __global__ void __launch_bounds__(kReduceKBlock) k_reduce_k(KStage R, const SubS
           if (v[k] == 0.0) continue;
           const long long e = e0 + k * bs;
           const int comp = (int)(e / np), j = (int)(e - (long long)comp * np);
+          if (j >= cnt) continue;
           const PairRec p = prY[j];
           const int col = comp < nc ? sh[p.a].bf_off + comp : sh[p.b].bf_off + comp - nc;
           __hip_atomic_fetch_add(&row[col], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -308,7 +311,9 @@ struct sqjk_ctx {
   struct Group { int begin, count, npp; };
   std::vector<Group> groups[kNPairClass];
   // work plan for the cut-offs (thr, thr64) of the last build
-  struct Launch { int X, Y, fp32, seg0, tile0; long long ntile, quartets; };
+  // lr0, nlr: its (chunk, tile begin, tile end) ranges in h_lrange when the
+  // exchange staging is held in ket-pair chunks
+  struct Launch { int X, Y, fp32, seg0, tile0; long long ntile, quartets; int lr0, nlr; };
   // plan_ok: the device plan (d_seg ... d_rhi) holds the host plan below,
   // made for the cut-offs (plan_thr, plan_thr64); hplan_ok: the host plan
   // was made for (hplan_thr, hplan_thr64)
@@ -325,8 +330,21 @@ struct sqjk_ctx {
   long long plan_q = 0, plan_q32 = 0;
   // pair-indexed digestion buffers per class: Jp [NAB][np], S [n][NC+ND][np]
   bool profile = false;
-  std::vector<hipEvent_t> prof_ev;                      // 2 per launch
+  std::vector<hipEvent_t> prof_ev;                      // 2 per recorded launch
+  std::vector<int> prof_launch;                         // launch of each event pair
   double cls_ms[kNPairClass][kNPairClass] = {};
+  // Exchange staging in ket-pair chunks, when the staging of all classes
+  // would take more than stage_budget bytes per spin: the launches of ket
+  // class Y run chunk by chunk (chunk_len[Y] pairs each, held with a stride
+  // of chunk_stride[Y] columns so that a tile starting in the chunk fits),
+  // sharing one buffer per spin (d_SC, d_SCB, sc_len doubles), and each
+  // chunk is folded into A_K before the next.
+  size_t stage_budget = size_t(256) << 20;
+  bool stage_chunked = false;
+  int chunk_len[kNPairClass] = {}, chunk_stride[kNPairClass] = {}, chunk_cnt[kNPairClass] = {};
+  double *d_SC = nullptr, *d_SCB = nullptr;
+  size_t sc_len = 0;
+  std::vector<std::array<int, 3>> h_lrange;
   double* d_Jp[kNPairClass] = {};
   double* d_SA[kNPairClass] = {};
   double* d_SB[kNPairClass] = {};
@@ -614,9 +632,54 @@ void stage_lengths(sqjk_ctx* c) {
   }
 }
 
+// Exchange staging budget per spin: 256 MiB, or SQJK_STAGE_MB.
+size_t stage_budget_bytes() {
+  if (const char* e = std::getenv("SQJK_STAGE_MB")) {
+    const long long mb = std::atoll(e);
+    if (mb > 0) return (size_t)mb << 20;
+  }
+  return size_t(256) << 20;
+}
+
+// Whether the exchange staging is held in ket-pair chunks, and their sizes:
+// whole classes when all of it fits in stage_budget bytes per spin, else
+// per ket class the longest chunk (a multiple of 64 pairs) whose buffer of
+// n x ncomp x (chunk + 64) doubles fits.
+void plan_stage(sqjk_ctx* c) {
+  stage_lengths(c);
+  size_t total = 0;
+  for (int X = 0; X < kNPairClass; ++X) total += c->s_len[X] * sizeof(double);
+  c->stage_chunked = total > c->stage_budget;
+  c->sc_len = 0;
+  for (int Y = 0; Y < kNPairClass; ++Y) {
+    const int np = (int)c->pairs[Y].size();
+    const size_t ncomp = sqmd::ncart(kPairL[Y][0]) + sqmd::ncart(kPairL[Y][1]);
+    if (!c->stage_chunked || np == 0) {
+      c->chunk_len[Y] = c->chunk_stride[Y] = np;
+      c->chunk_cnt[Y] = np > 0 ? 1 : 0;
+      continue;
+    }
+    const long long cols = (long long)(c->stage_budget / (sizeof(double) * (size_t)c->n * ncomp));
+    int len = (int)std::max<long long>(64, (cols - 64) / 64 * 64);
+    if (len >= np) len = np;
+    c->chunk_len[Y] = len;
+    c->chunk_stride[Y] = len >= np ? np : len + 64;
+    c->chunk_cnt[Y] = (np + len - 1) / len;
+    c->sc_len = std::max(c->sc_len, (size_t)c->n * ncomp * c->chunk_stride[Y]);
+  }
+}
+
 // Pair-indexed digestion buffers (alpha always, beta for UHF builds).
 int ensure_stage(sqjk_ctx* c, bool uhf) {
   stage_lengths(c);
+  if (c->stage_chunked) {
+    int rc;
+    for (int X = 0; X < kNPairClass; ++X)
+      if (!c->pairs[X].empty() && !c->d_Jp[X] && (rc = dalloc(&c->d_Jp[X], c->jp_len[X]))) return rc;
+    if (!c->d_SC && (rc = dalloc(&c->d_SC, c->sc_len))) return rc;
+    if (uhf && !c->d_SCB && (rc = dalloc(&c->d_SCB, c->sc_len))) return rc;
+    return SQJK_OK;
+  }
   for (int X = 0; X < kNPairClass; ++X) {
     if (c->pairs[X].empty()) continue;
     int rc;
@@ -771,6 +834,7 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
   c->hplan_thr = thr;
   c->hplan_thr64 = thr64;
   c->launches.clear();
+  c->h_lrange.clear();
   c->h_seg.clear();
   c->h_tile.clear();
   c->h_rbra.clear();
@@ -873,6 +937,22 @@ void build_plan(sqjk_ctx* c, double thr, double thr64) {
         }
         L.ntile = (long long)c->h_tile.size() - L.tile0;
         if (L.ntile == 0) continue;
+        if (c->stage_chunked) {
+          // tiles by the chunk of their first ket pair, one range per chunk
+          const int cl = c->chunk_len[Y];
+          auto chunk_of = [&](const Tile& t) { return (c->h_seg[t.seg].ket0 + t.k0) / cl; };
+          std::stable_sort(c->h_tile.begin() + L.tile0, c->h_tile.end(),
+                           [&](const Tile& p, const Tile& q) { return chunk_of(p) < chunk_of(q); });
+          L.lr0 = (int)c->h_lrange.size();
+          for (int t = L.tile0; t < (int)c->h_tile.size();) {
+            int e = t;
+            const int ch = chunk_of(c->h_tile[t]);
+            while (e < (int)c->h_tile.size() && chunk_of(c->h_tile[e]) == ch) ++e;
+            c->h_lrange.push_back({ch, t - L.tile0, e - L.tile0});
+            t = e;
+          }
+          L.nlr = (int)c->h_lrange.size() - L.lr0;
+        }
         c->launches.push_back(L);
         c->cls_q[X][Y] += L.quartets;
         c->plan_q += L.quartets;
@@ -941,6 +1021,7 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
   c->n = nBasis;
   init_tuning(c);
   if (const char* e = std::getenv("SQJK_DD_KERNEL")) c->no_wd = std::string(e) == "wave";
+  c->stage_budget = stage_budget_bytes();
   std::vector<double> pexp, pcoef;
   int rc = plan_shells(c, nBasis, gto, pexp, pcoef);
   if (rc) { delete c; return rc; }
@@ -968,6 +1049,7 @@ int sqjk_create(int nBasis, const GTOBasis_t* gto, const double* schwarz_basis, 
     return fail(SQJK_EHIP, "primitive pair upload failed");
   }
   if ((rc = build_herm_tables(c))) { sqjk_destroy(c); return rc; }
+  plan_stage(c);
   if (hipStreamSynchronize(c->st) != hipSuccess) { sqjk_destroy(c); return fail(SQJK_EHIP, "sync failed"); }
   *out = c;
   return SQJK_OK;
@@ -983,12 +1065,16 @@ long long sqjk_footprint_bytes(int nBasis, const GTOBasis_t* gto, int uhf, int w
   long long b = (long long)t.sub.size() * sizeof(SubShell) + (long long)t.pp.size() * sizeof(PrimPair) +
                 (long long)kBoysNT * kBoysCols * 8 + (long long)nBasis * 8 + (long long)t.nref * t.nref * 8;
   b += 10 * nn * 8;  // densities, half-accumulators, results, Schwarz
+  t.n = nBasis;
+  t.stage_budget = stage_budget_bytes();
+  plan_stage(&t);
   for (int X = 0; X < kNPairClass; ++X) {
     const long long np = (long long)t.pairs[X].size();
     const int nab = sqmd::ncart(kPairL[X][0]) * sqmd::ncart(kPairL[X][1]);
-    const int ncomp = sqmd::ncart(kPairL[X][0]) + sqmd::ncart(kPairL[X][1]);
-    b += np * sizeof(PairRec) + np * nab * 8 + (uhf ? 2 : 1) * (long long)nBasis * ncomp * np * 8;
+    b += np * sizeof(PairRec) + np * nab * 8;
+    if (!t.stage_chunked) b += (uhf ? 2 : 1) * (long long)t.s_len[X] * 8;
   }
+  if (t.stage_chunked) b += (uhf ? 2 : 1) * (long long)t.sc_len * 8;
   for (const PairRec& pr : t.pairs[kNPairClass - 1]) b += (long long)pr.npp * HermTabLayout<2, 2>::SIZE * 8;
   if (with_scf) b += 38 * nn * 8;  // SCF matrices, DIIS history, eigen-solver work
   return b + b / 8;                 // work plan and allocator slack
@@ -1221,35 +1307,47 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
     if (c->pairs[X].empty()) continue;
     hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)c->jp_len[X], 256)), dim3(256), 0, c->st, B.Jp[X],
                        c->jp_len[X], 0.0);
+    if (c->stage_chunked && !region) continue;  // the chunk buffer is filled per chunk
     const int sg = std::max(1, (int)std::min<size_t>((c->s_len[X] + 255) / 256, 16384));
     hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, B.SA[X], c->s_len[X], 0.0);
     if (uhf) hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, B.SB[X], c->s_len[X], 0.0);
   }
   HIPCHK(hipGetLastError());
   int launches = 0;
-  if (prof && c->prof_ev.size() < 2 * c->launches.size()) {
-    const size_t have = c->prof_ev.size();
-    c->prof_ev.resize(2 * c->launches.size());
-    for (size_t e = have; e < c->prof_ev.size(); ++e) HIPCHK(hipEventCreate(&c->prof_ev[e]));
-  }
-  for (size_t li = 0; li < c->launches.size(); ++li) {
+  const bool chunked = c->stage_chunked && !region;
+  c->prof_launch.clear();
+  // one class launch over the tiles [tb, te) of launch li, with the
+  // exchange staging SA/SB holding the ket pairs from s_k0 with stride s_np
+  auto launch = [&](size_t li, long long tb, long long te, double* SA, double* SB, int s_k0, int s_np) -> int {
     const sqjk_ctx::Launch& L = c->launches[li];
-    if (prof) HIPCHK(hipEventRecord(c->prof_ev[2 * li], c->st));
+    size_t ev = 0;
+    if (prof) {
+      ev = 2 * c->prof_launch.size();
+      c->prof_launch.push_back((int)li);
+      while (c->prof_ev.size() < ev + 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+      }
+      HIPCHK(hipEventRecord(c->prof_ev[ev], c->st));
+    }
     ClassArgs a = base_args(c);
     a.bra = c->d_pairs[L.X];
     a.ket = c->d_pairs[L.Y];
     a.seg = B.seg;
-    a.tile = B.tile + L.tile0;
+    a.tile = B.tile + L.tile0 + tb;
     a.rbra = B.rbra;
     a.rlo = B.rlo;
     a.rhi = B.rhi;
     a.same = (L.X == L.Y);
-    a.total = L.ntile;
+    a.total = te - tb;
     a.fixed_cut = fixedCutoff;
     a.JpB = B.Jp[L.X];
     a.JpK = B.Jp[L.Y];
-    a.SA = B.SA[L.Y];
-    a.SB = B.SB[L.Y];
+    a.SA = SA;
+    a.SB = SB;
+    a.s_k0 = s_k0;
+    a.s_np = s_np;
     a.npb = (int)c->pairs[L.X].size();
     a.npk = (int)c->pairs[L.Y].size();
     if (L.X == kNPairClass - 1 && L.Y == kNPairClass - 1 && c->d_herm && !c->no_wd) {
@@ -1260,7 +1358,66 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
     const int k = f ? f(a, uhf ? kUHF : kRHF, L.fp32, -c->grid_over[L.X][L.Y], c->st) : -1;
     if (k < 0) return fail(SQJK_EHIP, "class kernel launch failed");
     launches += k;
-    if (prof) HIPCHK(hipEventRecord(c->prof_ev[2 * li + 1], c->st));
+    if (prof) HIPCHK(hipEventRecord(c->prof_ev[ev + 1], c->st));
+    return SQJK_OK;
+  };
+  auto stage_of = [&](int Y, double* S, double* SBp, int p0, int np, int cnt) {
+    KStage R{};
+    R.S[Y] = S;
+    R.pr[Y] = c->d_pairs[Y];
+    R.np[Y] = np;
+    R.p0[Y] = p0;
+    R.cnt[Y] = cnt;
+    R.nc[Y] = sqmd::ncart(kPairL[Y][0]);
+    R.ncomp[Y] = sqmd::ncart(kPairL[Y][0]) + sqmd::ncart(kPairL[Y][1]);
+    (void)SBp;
+    return R;
+  };
+  if (!chunked) {
+    for (size_t li = 0; li < c->launches.size(); ++li) {
+      const sqjk_ctx::Launch& L = c->launches[li];
+      if ((rc = launch(li, 0, L.ntile, B.SA[L.Y], B.SB[L.Y], 0, (int)c->pairs[L.Y].size()))) return rc;
+    }
+  } else {
+    // ket class by ket class, chunk by chunk: fill, launch the tiles whose
+    // first ket pair is in the chunk, fold into A_K
+    for (int Y = 0; Y < kNPairClass; ++Y) {
+      const int np = (int)c->pairs[Y].size();
+      if (np == 0) continue;
+      const size_t ncomp = sqmd::ncart(kPairL[Y][0]) + sqmd::ncart(kPairL[Y][1]);
+      const int cl = c->chunk_len[Y], stride = c->chunk_stride[Y];
+      const size_t len = (size_t)n * ncomp * stride;
+      const int sg = std::max(1, (int)std::min<size_t>((len + 255) / 256, 16384));
+      for (int ch = 0; ch < c->chunk_cnt[Y]; ++ch) {
+        bool any = false;
+        for (const sqjk_ctx::Launch& L : c->launches)
+          if (L.Y == Y)
+            for (int r = L.lr0; r < L.lr0 + L.nlr; ++r) any = any || c->h_lrange[r][0] == ch;
+        if (!any) continue;
+        hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SC, len, 0.0);
+        if (uhf) hipLaunchKernelGGL(k_fill, dim3(sg), dim3(256), 0, c->st, c->d_SCB, len, 0.0);
+        HIPCHK(hipGetLastError());
+        for (size_t li = 0; li < c->launches.size(); ++li) {
+          const sqjk_ctx::Launch& L = c->launches[li];
+          if (L.Y != Y) continue;
+          for (int r = L.lr0; r < L.lr0 + L.nlr; ++r) {
+            const auto& lr = c->h_lrange[r];
+            if (lr[0] != ch) continue;
+            if ((rc = launch(li, lr[1], lr[2], c->d_SC, c->d_SCB, ch * cl, stride))) return rc;
+          }
+        }
+        const int p0 = ch * cl, cnt = std::min(stride, np - p0);
+        const KStage RA = stage_of(Y, c->d_SC, nullptr, p0, stride, cnt);
+        hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RA,
+                           c->d_sub, c->d_AKA, n);
+        if (uhf) {
+          const KStage RB = stage_of(Y, c->d_SCB, nullptr, p0, stride, cnt);
+          hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RB,
+                             c->d_sub, c->d_AKB, n);
+        }
+        HIPCHK(hipGetLastError());
+      }
+    }
   }
   // fold the pair-indexed buffers into the half-accumulators
   KStage RA{}, RB{};
@@ -1274,14 +1431,17 @@ int build_on_device(sqjk_ctx* c, double fixedCutoff, double scfCutoff, int uhf, 
     RB.S[X] = B.SB[X];
     RA.pr[X] = RB.pr[X] = c->d_pairs[X];
     RA.np[X] = RB.np[X] = np;
+    RA.cnt[X] = RB.cnt[X] = np;
     RA.nc[X] = RB.nc[X] = sqmd::ncart(kPairL[X][0]);
     RA.ncomp[X] = RB.ncomp[X] = sqmd::ncart(kPairL[X][0]) + nb;
   }
-  hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RA, c->d_sub,
-                     c->d_AKA, n);
-  if (uhf)
-    hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RB, c->d_sub,
-                       c->d_AKB, n);
+  if (!chunked) {
+    hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RA, c->d_sub,
+                       c->d_AKA, n);
+    if (uhf)
+      hipLaunchKernelGGL(k_reduce_k, dim3(n), dim3(kReduceKBlock), (size_t)n * sizeof(double), c->st, RB, c->d_sub,
+                         c->d_AKB, n);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev1, c->st));
   hipLaunchKernelGGL(k_finalize, dim3(fill_grid), dim3(256), 0, c->st, c->d_AJ, c->d_AKA, c->d_AKB, oJ, oKA, oKB,
@@ -1305,10 +1465,11 @@ int collect_times(sqjk_ctx* c) {
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   std::memset(c->cls_ms, 0, sizeof(c->cls_ms));
   if (c->profile)
-    for (size_t li = 0; li < c->launches.size(); ++li) {
+    for (size_t k = 0; k < c->prof_launch.size(); ++k) {
       float lm = 0.f;
-      HIPCHK(hipEventElapsedTime(&lm, c->prof_ev[2 * li], c->prof_ev[2 * li + 1]));
-      c->cls_ms[c->launches[li].X][c->launches[li].Y] += lm;
+      HIPCHK(hipEventElapsedTime(&lm, c->prof_ev[2 * k], c->prof_ev[2 * k + 1]));
+      const sqjk_ctx::Launch& L = c->launches[c->prof_launch[k]];
+      c->cls_ms[L.X][L.Y] += lm;
     }
   c->last.kernel_ms = ms;
   c->timing_pending = false;
@@ -1637,6 +1798,8 @@ void sqjk_destroy(sqjk_ctx* c) {
     block_put(p);
   block_put(c->d_ref);
   block_put(c->d_herm);
+  block_put(c->d_SC);
+  block_put(c->d_SCB);
   double* pins[] = {c->hp_A, c->hp_B, c->hp_J, c->hp_KA, c->hp_KB};
   for (double* h : pins)
     block_put(h);
@@ -1897,8 +2060,11 @@ int batch_plan(sqjk_batch* b) {
                 seg.push_back(sgm);
               }
               const long long t_end = l.tile0 + l.ntile;
-              const int r_first = c->h_tile[l.tile0].r0;
-              const int r_last = c->h_tile[t_end - 1].r1;
+              int r_first = INT32_MAX, r_last = 0;
+              for (long long t = l.tile0; t < t_end; ++t) {
+                r_first = std::min(r_first, c->h_tile[t].r0);
+                r_last = std::max(r_last, c->h_tile[t].r1);
+              }
               for (int r = r_first; r < r_last; ++r) {
                 rbra.push_back(c->h_rbra[r] + p0[i][X]);
                 rlo.push_back(c->h_rlo[r]);

@@ -111,9 +111,13 @@ namespace sqla {
 namespace {
 int read_info(Work& w) {
   rocblas_int info = 0;
-  if (hipMemcpyAsync(&info, w.info, sizeof(info), hipMemcpyDeviceToHost, w.st) != hipSuccess ||
-      hipStreamSynchronize(w.st) != hipSuccess)
+  hipError_t e = hipMemcpyAsync(&info, w.info, sizeof(info), hipMemcpyDeviceToHost, w.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.st);
+  if (e != hipSuccess) {
+    w.last_hip = e;
     return -3;
+  }
+  w.last_info = info;
   return info != 0 ? -5 : 0;
 }
 
@@ -170,10 +174,21 @@ void release(Work& w) {
   w = Work{};
 }
 
+const char* what(const Work& w, int rc) {
+  static thread_local char buf[160];
+  switch (rc) {
+    case -3: std::snprintf(buf, sizeof(buf), "device copy or synchronisation failed: %s", hipGetErrorString(w.last_hip)); break;
+    case -4: std::snprintf(buf, sizeof(buf), "rocBLAS/rocSOLVER call failed"); break;
+    case -5: std::snprintf(buf, sizeof(buf), "factorisation info %d (matrix not positive definite)", w.last_info); break;
+    default: std::snprintf(buf, sizeof(buf), "linear algebra error %d", rc); break;
+  }
+  return buf;
+}
+
 int set_metric(Work& w, int n, const double* dS) {
   const size_t nn = (size_t)n * n;
   w.metric_n = 0;
-  if (hipMemcpyAsync(w.X, dS, nn * 8, hipMemcpyDeviceToDevice, w.st) != hipSuccess) return -3;
+  if ((w.last_hip = hipMemcpyAsync(w.X, dS, nn * 8, hipMemcpyDeviceToDevice, w.st)) != hipSuccess) return -3;
   if (rocsolver_dpotrf(w.h, rocblas_fill_upper, n, w.X, n, w.info) != rocblas_status_success) return -4;
   if (int rc = read_info(w)) return rc;
   if (rocsolver_dtrtri(w.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, w.X, n, w.info) !=

@@ -23,7 +23,11 @@ struct Work {
   double* red = nullptr;            // Gershgorin bounds and traces (4)
   rocblas_int* info = nullptr;
   int metric_n = 0;                 // X = U^-1 holds the metric of this n
+  int last_info = 0;                // LAPACK-style info of the last factorisation
+  hipError_t last_hip = hipSuccess; // HIP error of the last failed call
 };
+// Text of the last failure of w (return code rc of the call).
+const char* what(const Work& w, int rc);
 
 // Buffers for size n on stream st (nullptr: a stream of its own).
 int ensure(Work& w, int n, hipStream_t st = nullptr);

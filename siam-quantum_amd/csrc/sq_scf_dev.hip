@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/sq_jk.h"
@@ -339,6 +340,9 @@ int sqscf_set_system(sqscf_ctx* c, int n, const struct GTOBasis_t* gto, const do
   if (c->jk && c->n == n && sqjk_update_geometry(c->jk, n, gto) == SQJK_OK) {
     // same basis at a new geometry: the J/K context is kept
   } else {
+    // the linear-algebra workspace (its rocBLAS handle) is bound to the old
+    // context's stream: release it while that stream still exists
+    if (c->lw.st) sqla::release(c->lw);
     if (c->jk) sqjk_destroy(c->jk);
     c->jk = nullptr;
     if ((rc = sqjk_create(n, gto, nullptr, c->device, &c->jk))) return scf_fail(rc, sqjk_errmsg());
@@ -353,7 +357,9 @@ int sqscf_set_system(sqscf_ctx* c, int n, const struct GTOBasis_t* gto, const do
   const size_t nn = (size_t)n * n;
   SCFCHK(hipMemcpyAsync(c->H, H, nn * 8, hipMemcpyHostToDevice, c->st));
   SCFCHK(hipMemcpyAsync(c->S, S, nn * 8, hipMemcpyHostToDevice, c->st));
-  if (sqla::set_metric(c->lw, n, c->S)) return scf_fail(SQJK_EHIP, "overlap matrix is not positive definite");
+  if (int e = sqla::set_metric(c->lw, n, c->S))
+    return scf_fail(SQJK_EHIP, std::string("metric from the overlap matrix (n = ") + std::to_string(n) + "): " +
+                                   sqla::what(c->lw, e));
   SCFCHK(hipStreamSynchronize(c->st));
   return SQJK_OK;
 }

@@ -153,7 +153,8 @@ void serve_one(sqsrv* s, Request& r, sqscf_ctx* scf) {
     free_basis(gto, n);
     if (rc) {
       r.error = rc;
-      r.errmsg = sqscf_errmsg();
+      r.errmsg = std::string("step ") + std::to_string(k) + " (" + std::to_string(n) + " functions): " +
+                 (rc == SQJK_OK ? "" : (sqscf_errmsg()[0] ? sqscf_errmsg() : sqjk_errmsg()));
       return;
     }
     r.energy[k] = res.converged ? res.energy : 0.0;

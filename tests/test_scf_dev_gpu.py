@@ -122,3 +122,20 @@ def test_update_geometry_matches_fresh_context():
     bas3.arr[0].exp[0] *= 1.01
     assert L.sqjk_update_geometry(eng._h, bas3.n, bas3.ptr) != 0
     eng.close()
+
+
+def test_context_switches_basis():
+    """One SCF context serving molecules of different bases in turn (as a
+    serving slot does): each solve equals uhf() on its own."""
+    scf = DeviceSCF()
+    for name, basis in (("h2o_q0m1", "631g_star"), ("c2h4_q0m1", "631g_star"), ("h2o_q0m1", "sto3g"),
+                        ("ch2o_q0m1", "631g_star")):
+        mol, bas = g2(name, basis)
+        ne = mol.n_electrons
+        _, S = scf.set_system(bas, mol)
+        scf.diag_guess(S, ne // 2, ne // 2)
+        r = scf.run(ne // 2, ne // 2, conv=1e-10, three_step=False)
+        E = _uhf(mol, bas, ne // 2, ne // 2, SCFConv=1e-10, SCFAccuracy=1)
+        print(f"{name}/{basis} n={bas.n}: {r['energy']:.10f} ({r['iterations']} it), uhf() {E:.10f}")
+        assert r["converged"] and abs(r["energy"] - E) < 1e-8
+    scf.close()
