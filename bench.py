@@ -108,14 +108,15 @@ def scf_solves(name, nsolve, mixed_thr):
 
 
 def profile_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE runs, calibrated against a plain
-    copy), or None when no summary names this kernel."""
+    """HBM-side bytes of `kernel` per build (all of its launches in one
+    build) from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE runs, calibrated against a plain copy), or None when no
+    summary names this kernel."""
     path = os.path.join(PROFILES, "jk_c60_631gs_hbm_traffic.md")
     if not os.path.exists(path):
         return None, None
     for line in open(path):
-        m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+)\s+(\d+)", line)
+        m = re.match(r"\s*traffic_bytes_per_launch\s*\|\s*(.+?)\s*\|\s*(\d+)\s*\|\s*(\d+)", line)
         if m and m.group(1) == kernel:
             return int(m.group(2)) + int(m.group(3)), os.path.relpath(path, ROOT)
     return None, None
