@@ -61,8 +61,10 @@ def uhf_fn():
     return L
 
 
-def scf_once(mol, bas, **kw):
-    """RHF through the uhf() drop-in; returns (E, density P_alpha, seconds)."""
+def scf_once(mol, bas, orbitals=None, **kw):
+    """RHF through the uhf() drop-in; returns (E, density P_alpha, seconds).
+    With a dict `orbitals`, the orbitals, orbital energies and options are
+    stored in it (CA, CB, eA, eB, opt)."""
     from sqamd import default_option, dptr
     L = uhf_fn()
     n, ne = bas.n, mol.n_electrons
@@ -71,8 +73,46 @@ def scf_once(mol, bas, **kw):
     t0 = time.perf_counter()
     E = L.uhf(n, bas.ptr, mol.ptr, ne // 2, ne // 2, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt))
     dt = time.perf_counter() - t0
+    if orbitals is not None:
+        orbitals.update(CA=CA, CB=CB, eA=eA, eB=eB, opt=opt)
     occ = CA[:ne // 2]
     return E, occ.T @ occ, dt
+
+
+def forces_bench(eng, mol, bas, P, orbs, steps):
+    """Hartree-Fock forces of the converged C60 SCF: the two-electron
+    gradient (sqjk_grad_eri on the build context, cut-off 1e-10 as the
+    reference's GRAD_SCHWARZ_CUTOFF), the one-electron gradient
+    (sqjk_grad_one_electron) and the whole uhf_force drop-in (host densities,
+    energy-weighted density, both device parts, host arrays in and out)."""
+    from sqamd.engine import basis_to_atom, grad_one_electron, uhf_force
+    n, na = bas.n, mol.ptr.contents.nAtom
+    b2a = basis_to_atom(mol, bas)
+    nocc = mol.n_electrons // 2
+    CA, eA = orbs["CA"], orbs["eA"]
+    QT = 2.0 * (CA[:nocc].T * eA[:nocc]) @ CA[:nocc]
+    eng.set_profile(False)
+    eng.grad_eri(P, P, b2a, na, 1.0, 1e-10)  # warm-up (plan for the gradient cut-off)
+    t2, k2 = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        G2 = eng.grad_eri(P, P, b2a, na, 1.0, 1e-10)
+        t2.append(time.perf_counter() - t0)
+        k2.append(eng.stats()["kernel_ms"])
+    grad_one_electron(bas, mol, 2 * P, QT, b2a)
+    t1 = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        G1 = grad_one_electron(bas, mol, 2 * P, QT, b2a)
+        t1.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    F = uhf_force(bas, mol, nocc, nocc, orbs["CA"], orbs["CB"], orbs["eA"], orbs["eB"], orbs["opt"])
+    tf = time.perf_counter() - t0
+    return {"workload": "C60 RHF/6-31G* forces on the converged SCF (SCFConv 1e-6), FP64",
+            "two_electron_ms": 1e3 * float(np.median(t2)), "two_electron_kernel_ms": float(np.median(k2)),
+            "one_electron_ms": 1e3 * float(np.median(t1)), "uhf_force_ms": 1e3 * tf,
+            "max_abs_force": float(np.abs(F).max()), "sum_forces": float(np.abs(F.sum(0)).max()),
+            "max_abs_grad_two_electron": float(np.abs(G2).max()), "max_abs_grad_one_electron": float(np.abs(G1).max())}
 
 
 def scf_solves(name, nsolve, mixed_thr):
@@ -294,7 +334,8 @@ def main():
     mol, bas, desc = molecule("c60")
     n = bas.n
     print(f"bench: C60 SCF for the step density (n={n})", file=sys.stderr, flush=True)
-    E_scf, P, t_scf = scf_once(mol, bas, SCFConv=1e-6)
+    orbs = {}
+    E_scf, P, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-6)
     print(f"bench: C60 E = {E_scf:.8f} Eh in {t_scf:.1f} s; timing builds", file=sys.stderr, flush=True)
     eng = JKEngine(bas.ptr, n)
     eng.set_profile(True)
@@ -385,6 +426,9 @@ def main():
                             "kernel_ms": float(np.mean(mk)), "quartets_fp32": stm["quartets_fp32"],
                             "max_abs_dJ": float(np.abs(Jm - J64).max()), "max_abs_dK": float(np.abs(Km - K64).max()),
                             "speedup_vs_fp64": (a.steps / wall_m) / out["value"]}
+    if not a.no_extras:
+        print("bench: C60 forces", file=sys.stderr, flush=True)
+        out["forces"] = forces_bench(eng, mol, bas, P, orbs, a.steps)
     eng.close()
     if not a.no_extras:
         mol32, bas32, desc32 = molecule("w32")

@@ -203,7 +203,7 @@ int sqjk_class_times(const sqjk_ctx *ctx, double *kernel_ms);
 int sqjk_one_electron(int nBasis, const struct GTOBasis_t *gto, const struct Molecule_t *mol, int device,
                       double *T, double *V, double *S);
 
-/* Boys function F_0..F_L (L <= 8) at nT points T >= 0, evaluated on the
+/* Boys function F_0..F_L (L <= 9) at nT points T >= 0, evaluated on the
  * device by the integral kernels' own code (LDS table with Taylor expansion
  * and downward recursion below the per-L cut-off, sqrt(pi/T)/2 with upward
  * recursion above it), in FP32 when fp32 is set.  F[i*(L+1) + m] = F_m(T_i).

@@ -69,6 +69,18 @@ class Oracle:
         L.or_gen_eigen.restype = I
         L.or_uhf.argtypes = [I, PG, PM, I, I, I, D, I, D, I, I, I, I, PD, PD, PD, PD, C.POINTER(I)]
         L.or_uhf.restype = D
+        for f in ("or_grad_overlap", "or_grad_kinetic"):
+            getattr(L, f).argtypes = [I, I, PG, I]
+            getattr(L, f).restype = D
+        L.or_grad_nai.argtypes = [I, I, PG, PM, I]
+        L.or_grad_nai.restype = D
+        L.or_grad_moment.argtypes = [I, I, PG, I, I, I, D, D, D, I]
+        L.or_grad_moment.restype = D
+        L.or_gto_efi.argtypes = [I, I, PG, D, D, D, PD, PD, PD]
+        L.or_grad_vnn.argtypes = [PM, PD, PD, PD]
+        L.or_grad_eri.argtypes = [I, I, I, PG, C.POINTER(I), PD, PD, D, PD, PD, PD]
+        L.or_grad_eri_cut.argtypes = [I, I, I, PG, C.POINTER(I), PD, PD, D, D, PD, PD, PD]
+        L.or_uhf_force.argtypes = [I, PG, PM, I, I, PD, PD, PD, PD, D, D, D, D, PD, PD, PD]
 
     # -- scalar pieces -------------------------------------------------
     def boys(self, mmax, T):
@@ -131,14 +143,63 @@ class Oracle:
         return J, KA, KB
 
     def uhf(self, g, n, mol, nEA, nEB, restricted=True, scf_conv=1e-8, scf_max=80, drag=0.25,
-            three_step=True, guess=0, use_mpole=True, max_mem=250):
+            three_step=True, guess=0, use_mpole=True, max_mem=250, want_orbitals=False):
         CA, CB = np.zeros((n, n)), np.zeros((n, n))
         eA, eB = np.zeros(n), np.zeros(n)
         it = I()
         E = self.L.or_uhf(n, g, mol, nEA, nEB, int(restricted), scf_conv, scf_max, drag,
                           int(three_step), guess, int(use_mpole), max_mem,
                           dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(it))
+        if want_orbitals:
+            return E, it.value, (CA, CB, eA, eB)
         return E, it.value
+
+    # -- forces --------------------------------------------------------
+    def grad_eri(self, g, n, basis2atom, natom, PT, PS, kfactor=1.0, child=0, ncpu=1, cutoff=None):
+        """Two-electron energy gradient per atom, [natom, 3]; cutoff None is
+        the reference's 1e-10, 0 the exact derivative."""
+        b2a = np.ascontiguousarray(basis2atom, dtype=np.int32)
+        PT = np.ascontiguousarray(PT, dtype=np.float64)
+        PS = np.ascontiguousarray(PS, dtype=np.float64)
+        G = np.zeros((3, natom))
+        if cutoff is None:
+            self.L.or_grad_eri(child, ncpu, n, g, b2a.ctypes.data_as(C.POINTER(I)), dptr(PT), dptr(PS), kfactor,
+                               dptr(G[0]), dptr(G[1]), dptr(G[2]))
+        else:
+            self.L.or_grad_eri_cut(child, ncpu, n, g, b2a.ctypes.data_as(C.POINTER(I)), dptr(PT), dptr(PS),
+                                   kfactor, cutoff, dptr(G[0]), dptr(G[1]), dptr(G[2]))
+        return G.T.copy()
+
+    def grad_one_electron(self, g, n, mol, basis2atom, PT, QT):
+        """One-electron energy gradient per atom, [natom, 3]: the loops of
+        src/grad.c:2236-2306 over the oracle's derivative integrals."""
+        m = mol.contents
+        na = m.nAtom
+        G = np.zeros((na, 3))
+        for p in range(n):
+            for q in range(n):
+                for d in range(3):
+                    G[basis2atom[p], d] -= 2.0 * self.L.or_grad_overlap(p, q, g, d) * QT[p, q]
+                    G[basis2atom[p], d] += 2.0 * self.L.or_grad_kinetic(p, q, g, d) * PT[p, q]
+                    G[basis2atom[p], d] += 2.0 * self.L.or_grad_nai(p, q, g, mol, d) * PT[p, q]
+        e = [D() for _ in range(3)]
+        for p in range(n):
+            for q in range(p + 1):
+                for i in range(na):
+                    self.L.or_gto_efi(p, q, g, m.x[i], m.y[i], m.z[i], *[C.byref(v) for v in e])
+                    w = (2.0 if p != q else 1.0) * m.Z[i] * PT[p, q]
+                    for d in range(3):
+                        G[i, d] += e[d].value * w
+        return G
+
+    def force(self, g, n, mol, nEA, nEB, orbitals, field=(0.0, 0.0, 0.0), kfactor=1.0):
+        """Hartree-Fock forces -dE/dR per atom, [natom, 3]."""
+        CA, CB, eA, eB = (np.ascontiguousarray(a, dtype=np.float64) for a in orbitals)
+        na = mol.contents.nAtom
+        F = np.zeros((3, na))
+        self.L.or_uhf_force(n, g, mol, nEA, nEB, dptr(CA), dptr(CB), dptr(eA), dptr(eB), *field, kfactor,
+                            dptr(F[0]), dptr(F[1]), dptr(F[2]))
+        return F.T.copy()
 
 
 def legacy_rhf(oracle, g, n, mol, nE, drag=0.25, scf_max=80):
@@ -193,6 +254,38 @@ class ReferenceCheck:
         L.load_checkpoint_basis.argtypes = [C.c_void_p, C.POINTER(I)]
         L.load_checkpoint_basis.restype = PG
         L.load_checkpoint_multiplicity.argtypes = [C.c_void_p]
+
+
+class ReferenceGrad:
+    """The reference's force routines (src/grad.c, compiled into
+    oracle/_ref/libsqref_grad.so).  Loaded with lazy binding: uhf_force's
+    unresolved symbols (uhf_getDMatrix, the RPC layer, the DFT grid) are
+    never reached from the functions bound here."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libsqref_grad.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+        L = self.L = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        for f in ("GTO_grad_overlap", "GTO_grad_kinetic"):
+            getattr(L, f).argtypes = [I, I, PG, I]
+            getattr(L, f).restype = D
+        L.GTO_grad_nai.argtypes = [I, I, PG, PM, I]
+        L.GTO_grad_nai.restype = D
+        L.GTO_grad_moment.argtypes = [I, I, PG, I, I, I, D, D, D, I]
+        L.GTO_grad_moment.restype = D
+        L.GTO_efi.argtypes = [I, I, PG, D, D, D, PD, PD, PD]
+        L.GradVnn.argtypes = [PM, PD, PD, PD]
+        L.GradEri_ShellSet_Parallel.argtypes = [I, I, I, PG, C.POINTER(I), PD, PD, D, PD, PD, PD]
+
+    def grad_eri(self, g, n, basis2atom, natom, PT, PS, kfactor=1.0, child=0, ncpu=1):
+        b2a = np.ascontiguousarray(basis2atom, dtype=np.int32)
+        PT = np.ascontiguousarray(PT, dtype=np.float64)
+        PS = np.ascontiguousarray(PS, dtype=np.float64)
+        G = np.zeros((3, natom))
+        self.L.GradEri_ShellSet_Parallel(child, ncpu, n, g, b2a.ctypes.data_as(C.POINTER(I)), dptr(PT), dptr(PS),
+                                         kfactor, dptr(G[0]), dptr(G[1]), dptr(G[2]))
+        return G.T.copy()
 
 
 class Reference:

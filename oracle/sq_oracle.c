@@ -1325,6 +1325,395 @@ double or_uhf(int n, const struct GTOBasis_t *g, const struct Molecule_t *mol, i
 	return notconv ? 0.0 : E;
 }
 
+/* ------------------------------------------------------------------ */
+/* Forces (src/grad.c)                                                 */
+/* ------------------------------------------------------------------ */
+
+#define OR_GRAD_CUTOFF 1.0e-10          /* GRAD_SCHWARZ_CUTOFF, src/grad.h:20 */
+
+/* nuclear repulsion gradient, added into G (src/grad.c:43-66) */
+void or_grad_vnn(const struct Molecule_t *m, double *Gx, double *Gy, double *Gz)
+{
+	for (int a = 0; a < m->nAtom; a++)
+		for (int b = 0; b < a; b++) {
+			double dx = m->x[a] - m->x[b], dy = m->y[a] - m->y[b], dz = m->z[a] - m->z[b];
+			double r = sqrt(dx * dx + dy * dy + dz * dz);
+			double f = m->Z[a] * m->Z[b] / (r * r * r);
+			Gx[a] -= f * dx; Gx[b] += f * dx;
+			Gy[a] -= f * dy; Gy[b] += f * dy;
+			Gz[a] -= f * dz; Gz[b] += f * dz;
+		}
+}
+
+/* one primitive pair of a one-electron operator with the angular powers of
+ * function i replaced by (l, m, n) */
+enum { OR_1E_S, OR_1E_T, OR_1E_V, OR_1E_M };
+typedef struct { const struct Molecule_t *mol; int mx, my, mz; double cx, cy, cz; } or_1e_op;
+
+static double or_1e_prim(int kind, const or_1e_op *op, const struct GTOBasis_t *g,
+                         int i, int a, int l, int m, int n, int j, int b)
+{
+	const struct GTOBasis_t *A = g + i, *B = g + j;
+	switch (kind) {
+	case OR_1E_S:
+		return or_overlap(A->exp[a], l, m, n, A->x0, A->y0, A->z0, B->exp[b], B->l, B->m, B->n, B->x0, B->y0, B->z0);
+	case OR_1E_T:
+		return or_kinetic(A->exp[a], l, m, n, A->x0, A->y0, A->z0, B->exp[b], B->l, B->m, B->n, B->x0, B->y0, B->z0);
+	case OR_1E_M:
+		return or_moment(A->exp[a], l, m, n, A->x0, A->y0, A->z0, B->exp[b], B->l, B->m, B->n, B->x0, B->y0, B->z0,
+		                 op->mx, op->my, op->mz, op->cx, op->cy, op->cz);
+	default: {
+		double s = 0.0;
+		for (int k = 0; k < op->mol->nAtom; k++)
+			s += (double)op->mol->Z[k] *
+			     or_nai(A->x0, A->y0, A->z0, 1.0, l, m, n, A->exp[a], B->x0, B->y0, B->z0, 1.0, B->l, B->m, B->n,
+			            B->exp[b], op->mol->x[k], op->mol->y[k], op->mol->z[k]);
+		return s;
+	}
+	}
+}
+
+/* derivative with respect to the centre of function i along dir (0,1,2) of
+ * a contracted one-electron integral: d/dX_i chi_i = 2 alpha chi_i(l+1) -
+ * l chi_i(l-1) (src/grad.c:89-160 overlap, 166-247 moment, 248-315 kinetic,
+ * 321-400 nuclear attraction, which differentiates the basis function only) */
+static double or_grad_1e(int kind, const or_1e_op *op, const struct GTOBasis_t *g, int i, int j, int dir)
+{
+	if (dir < 0 || dir > 2) { printf("or_grad_1e: error unrecognized direction flag\n"); exit(-1); }
+	int dl = dir == 0, dm = dir == 1, dn = dir == 2;
+	int lo = dl * g[i].l + dm * g[i].m + dn * g[i].n;
+	double down = 0.0, up = 0.0;
+	for (int a = 0; a < g[i].nContract; a++)
+		for (int b = 0; b < g[j].nContract; b++) {
+			double c = g[i].coef[a] * g[j].coef[b] * g[i].norm[a] * g[j].norm[b];
+			if (lo > 0) down += c * or_1e_prim(kind, op, g, i, a, g[i].l - dl, g[i].m - dm, g[i].n - dn, j, b);
+			up += g[i].exp[a] * c * or_1e_prim(kind, op, g, i, a, g[i].l + dl, g[i].m + dm, g[i].n + dn, j, b);
+		}
+	return -(down * lo) + up + up;
+}
+
+double or_grad_overlap(int i, int j, const struct GTOBasis_t *g, int dir)
+{
+	or_1e_op op = {0};
+	return or_grad_1e(OR_1E_S, &op, g, i, j, dir);
+}
+
+double or_grad_kinetic(int i, int j, const struct GTOBasis_t *g, int dir)
+{
+	or_1e_op op = {0};
+	return or_grad_1e(OR_1E_T, &op, g, i, j, dir);
+}
+
+double or_grad_nai(int i, int j, const struct GTOBasis_t *g, const struct Molecule_t *mol, int dir)
+{
+	or_1e_op op = {0};
+	op.mol = mol;
+	return or_grad_1e(OR_1E_V, &op, g, i, j, dir);
+}
+
+double or_grad_moment(int i, int j, const struct GTOBasis_t *g, int mx, int my, int mz,
+                      double cx, double cy, double cz, int dir)
+{
+	or_1e_op op = {0};
+	op.mx = mx; op.my = my; op.mz = mz; op.cx = cx; op.cy = cy; op.cz = cz;
+	return or_grad_1e(OR_1E_M, &op, g, i, j, dir);
+}
+
+/* Electric-field integral at C between contracted functions i and j
+ * (src/matrix.c:207-240 over src/int.c:469-525).  The attraction integral
+ * depends on A - C and B - C only, so its derivative with respect to C is
+ * minus the sum of its derivatives with respect to A and B.  The field
+ * integral is e = d/dC nai(i, j; C), with nai = -(i|1/|r-C||j). */
+void or_gto_efi(int i, int j, const struct GTOBasis_t *g, double xc, double yc, double zc,
+                double *ex, double *ey, double *ez)
+{
+	struct Molecule_t one;
+	int Z = 1;
+	double x = xc, y = yc, z = zc;
+	one.nAtom = 1; one.Q = 0; one.Z = &Z; one.x = &x; one.y = &y; one.z = &z;
+	or_1e_op op = {0};
+	op.mol = &one;
+	double e[3];
+	for (int d = 0; d < 3; d++)
+		e[d] = -(or_grad_1e(OR_1E_V, &op, g, i, j, d) + or_grad_1e(OR_1E_V, &op, g, j, i, d));
+	*ex = e[0]; *ey = e[1]; *ez = e[2];
+}
+
+/* the shells of the gradient loop: runs of consecutive functions with the
+ * same centre and exponents, no reordering (src/grad.c:982-1074) */
+static int or_grad_shells(int n, const struct GTOBasis_t *g, int *map, int *maxL)
+{
+	int ns = 1;
+	map[0] = 0;
+	maxL[0] = g[0].l + g[0].m + g[0].n;
+	for (int k = 1; k < n; k++) {
+		if (or_same_shell(g, k, map[ns - 1])) {
+			int L = g[k].l + g[k].m + g[k].n;
+			if (L > maxL[ns - 1]) maxL[ns - 1] = L;
+		} else {
+			map[ns] = k;
+			maxL[ns] = g[k].l + g[k].m + g[k].n;
+			ns++;
+		}
+	}
+	map[ns] = n;
+	return ns;
+}
+
+/* 1-D THO B arrays of one primitive quartet, computed on first use for the
+ * angular powers (l1, l2, l3, l4) of one axis */
+#define OR_GB_MAXL 4
+typedef struct {
+	double P[3], A[3], B[3], Q[3], Cc[3], D[3], eta1, eta2, zeta;
+	unsigned char have[3][OR_GB_MAXL][OR_GB_MAXL][OR_GB_MAXL][OR_GB_MAXL];
+	double Bv[3][OR_GB_MAXL][OR_GB_MAXL][OR_GB_MAXL][OR_GB_MAXL][4 * OR_GB_MAXL];
+} or_bcache;
+
+static const double *or_bget(or_bcache *c, int ax, int l1, int l2, int l3, int l4)
+{
+	double *B = c->Bv[ax][l1][l2][l3][l4];
+	if (!c->have[ax][l1][l2][l3][l4]) {
+		or_eriB_1d(B, l1, l2, l3, l4, c->P[ax], c->A[ax], c->B[ax], c->Q[ax], c->Cc[ax], c->D[ax],
+		           c->eta1, c->eta2, c->zeta);
+		c->have[ax][l1][l2][l3][l4] = 1;
+	}
+	return B;
+}
+
+/* (pq|ij) of one primitive quartet without the prefactor, for powers
+ * a[4][3] = (l,m,n) of p, q, i, j */
+static double or_bsum(or_bcache *c, const int a[4][3], const double *F)
+{
+	const double *Bx = or_bget(c, 0, a[0][0], a[1][0], a[2][0], a[3][0]);
+	const double *By = or_bget(c, 1, a[0][1], a[1][1], a[2][1], a[3][1]);
+	const double *Bz = or_bget(c, 2, a[0][2], a[1][2], a[2][2], a[3][2]);
+	int mx = a[0][0] + a[1][0] + a[2][0] + a[3][0];
+	int my = a[0][1] + a[1][1] + a[2][1] + a[3][1];
+	int mz = a[0][2] + a[1][2] + a[2][2] + a[3][2];
+	double s = 0.0;
+	for (int x = 0; x <= mx; x++)
+		for (int y = 0; y <= my; y++)
+			for (int z = 0; z <= mz; z++)
+				s += Bx[x] * By[y] * Bz[z] * F[x + y + z];
+	return s;
+}
+
+/* Two-electron part of the energy gradient, added into G per atom
+ * (src/grad.c:1734-2090, GradEri_ShellSet_Parallel): shell pairs pqS with
+ * pqS % nCPU == childID; shell quartets skipped when
+ * Sch_pq Sch_ij PM_pq PM_ij < 1e-10 (PM: max |PT| in the shell pair); per
+ * distinct basis quartet the density weight summed over its distinct
+ * permutations of 0.5 PT_pq PT_ij - Kfactor/4 (PT_pi PT_qj + PS_pi PS_qj);
+ * per primitive quartet the bare-prefactor cut 1e-15 and the per-term cut
+ * |weight * prefactor * coefficients| < 1e-10; derivatives with respect to
+ * the centres of p, q and i from raised and lowered powers, j's by
+ * translational invariance. */
+void or_grad_eri_cut(int childID, int nCPU, int n, const struct GTOBasis_t *g, const int *basis2Atom,
+                     const double *PT, const double *PS, double Kfactor, double cutoff,
+                     double *Gx, double *Gy, double *Gz)
+{
+	double *sch = malloc(sizeof(double) * n * n);
+	or_schwarz(n, g, sch);
+	int *map = malloc(sizeof(int) * (n + 1)), *maxL = malloc(sizeof(int) * (n + 1));
+	int ns = or_grad_shells(n, g, map, maxL);
+	for (int k = 0; k < ns; k++)
+		if (maxL[k] > OR_GB_MAXL - 2 || map[k + 1] - map[k] > 16) {
+			printf("or_grad_eri: shells above d are not restated\n");
+			exit(-1);
+		}
+	double *SS = calloc((size_t)ns * ns, sizeof(double)), *PM = calloc((size_t)ns * ns, sizeof(double));
+	for (int pS = 0; pS < ns; pS++)
+		for (int qS = 0; qS < ns; qS++) {
+			double r = 0.0, d = 0.0;
+			for (int p = map[pS]; p < map[pS + 1]; p++)
+				for (int q = map[qS]; q < map[qS + 1] && q < p + 1; q++) {
+					if (sch[p * n + q] > r) r = sch[p * n + q];
+					if (fabs(PT[p * n + q]) > d) d = fabs(PT[p * n + q]);
+				}
+			SS[pS * ns + qS] = r;
+			PM[pS * ns + qS] = d;
+		}
+	double *gx = calloc(n, sizeof(double)), *gy = calloc(n, sizeof(double)), *gz = calloc(n, sizeof(double));
+	double *gv[3] = {gx, gy, gz};
+	double *wt = malloc(sizeof(double) * 16 * 16 * 16 * 16);
+	or_bcache *bc = malloc(sizeof(or_bcache));
+	double kc = 0.25 * Kfactor;
+#define OR_W(a, b, c, d) (0.5 * PT[(a) * n + (b)] * PT[(c) * n + (d)] - \
+                          kc * (PT[(a) * n + (c)] * PT[(b) * n + (d)] + PS[(a) * n + (c)] * PS[(b) * n + (d)]))
+	for (int pqS = 0, pS = 0; pS < ns; pS++)
+	for (int qS = 0; qS <= pS; qS++, pqS++) {
+		if (pqS % nCPU != childID) continue;
+		for (int iS = 0; iS <= pS; iS++)
+		for (int jS = 0; jS <= pS; jS++) {
+			if (SS[pS * ns + qS] * SS[iS * ns + jS] * PM[pS * ns + qS] * PM[iS * ns + jS] < cutoff) continue;
+			/* density weights of the distinct basis quartets, in loop order */
+			int nq = 0;
+			for (int p = map[pS]; p < map[pS + 1]; p++)
+			for (int q = map[qS]; q < map[qS + 1] && q <= p; q++)
+			for (int i = map[iS]; i < map[iS + 1] && i <= p; i++)
+			for (int j = map[jS]; j < map[jS + 1]; j++) {
+				int lim = (i == p) ? q + 1 : i + 1;
+				if (j >= lim) break;
+				double w;
+				if (p == q && i == j && p == i) w = OR_W(p, q, i, j);
+				else if (p == q && i == j) w = OR_W(p, q, i, j) + OR_W(i, j, p, q);
+				else if (p == q) w = OR_W(p, q, i, j) + OR_W(p, q, j, i) + OR_W(i, j, p, q) + OR_W(j, i, p, q);
+				else if (i == j) w = OR_W(p, q, i, j) + OR_W(q, p, i, j) + OR_W(i, j, p, q) + OR_W(i, j, q, p);
+				else if (p == i && q == j) w = OR_W(p, q, i, j) + OR_W(q, p, i, j) + OR_W(p, q, j, i) + OR_W(q, p, j, i);
+				else w = OR_W(p, q, i, j) + OR_W(q, p, i, j) + OR_W(p, q, j, i) + OR_W(q, p, j, i) +
+				         OR_W(i, j, p, q) + OR_W(j, i, p, q) + OR_W(i, j, q, p) + OR_W(j, i, q, p);
+				wt[nq++] = w;
+			}
+			const struct GTOBasis_t *G0 = g + map[pS], *G1 = g + map[qS], *G2 = g + map[iS], *G3 = g + map[jS];
+			int Lsum = maxL[pS] + maxL[qS] + maxL[iS] + maxL[jS] + 2;
+			for (int a = 0; a < G0->nContract; a++)
+			for (int b = 0; b < G1->nContract; b++)
+			for (int c = 0; c < G2->nContract; c++)
+			for (int d = 0; d < G3->nContract; d++) {
+				double aa = G0->exp[a], ab = G1->exp[b], ac = G2->exp[c], ad = G3->exp[d];
+				double eta1 = 1.0 / (aa + ab), eta2 = 1.0 / (ac + ad);
+				double Ac[3] = {G0->x0, G0->y0, G0->z0}, Bc[3] = {G1->x0, G1->y0, G1->z0};
+				double Cc[3] = {G2->x0, G2->y0, G2->z0}, Dc[3] = {G3->x0, G3->y0, G3->z0};
+				double rab2 = 0, rcd2 = 0, rpq2 = 0;
+				for (int x = 0; x < 3; x++) {
+					bc->A[x] = Ac[x]; bc->B[x] = Bc[x]; bc->Cc[x] = Cc[x]; bc->D[x] = Dc[x];
+					bc->P[x] = (aa * Ac[x] + ab * Bc[x]) * eta1;
+					bc->Q[x] = (ac * Cc[x] + ad * Dc[x]) * eta2;
+					rab2 += (Ac[x] - Bc[x]) * (Ac[x] - Bc[x]);
+					rcd2 += (Cc[x] - Dc[x]) * (Cc[x] - Dc[x]);
+					rpq2 += (bc->P[x] - bc->Q[x]) * (bc->P[x] - bc->Q[x]);
+				}
+				bc->eta1 = eta1; bc->eta2 = eta2;
+				bc->zeta = 4.0 * (aa + ab) * (ac + ad) / (aa + ab + ac + ad);
+				double t = OR_TWO_PI_POW2_5 * eta1 * eta2 / sqrt(aa + ab + ac + ad) *
+				           exp(-aa * ab * rab2 * eta1 - ac * ad * rcd2 * eta2);
+				if (t < OR_PRIM_CUTOFF) continue;
+				double F[64];
+				or_fgamma_set(Lsum, 0.25 * rpq2 * bc->zeta, F);
+				memset(bc->have, 0, sizeof(bc->have));
+				int k = 0;
+				for (int p = map[pS]; p < map[pS + 1]; p++)
+				for (int q = map[qS]; q < map[qS + 1] && q <= p; q++)
+				for (int i = map[iS]; i < map[iS + 1] && i <= p; i++)
+				for (int j = map[jS]; j < map[jS + 1]; j++) {
+					int lim = (i == p) ? q + 1 : i + 1;
+					if (j >= lim) break;
+					double w = wt[k++];
+					double r = t * g[p].coef[a] * g[p].norm[a] * g[q].coef[b] * g[q].norm[b] *
+					           g[i].coef[c] * g[i].norm[c] * g[j].coef[d] * g[j].norm[d];
+					if (fabs(w * r) < cutoff) continue;
+					int fn[4] = {p, q, i, j};
+					double ex[3] = {aa, ab, ac};
+					for (int ax = 0; ax < 3; ax++) {
+						double dd[3];
+						for (int ctr = 0; ctr < 3; ctr++) {
+							int pw[4][3];
+							for (int f = 0; f < 4; f++) {
+								pw[f][0] = g[fn[f]].l; pw[f][1] = g[fn[f]].m; pw[f][2] = g[fn[f]].n;
+							}
+							int lo = pw[ctr][ax];
+							pw[ctr][ax] = lo + 1;
+							double v = 2.0 * ex[ctr] * or_bsum(bc, pw, F);
+							if (lo > 0) {
+								pw[ctr][ax] = lo - 1;
+								v -= lo * or_bsum(bc, pw, F);
+							}
+							dd[ctr] = v * r;
+						}
+						gv[ax][p] += w * dd[0];
+						gv[ax][q] += w * dd[1];
+						gv[ax][i] += w * dd[2];
+						gv[ax][j] -= w * (dd[0] + dd[1] + dd[2]);
+					}
+				}
+			}
+		}
+	}
+#undef OR_W
+	for (int p = 0; p < n; p++) {
+		Gx[basis2Atom[p]] += gx[p];
+		Gy[basis2Atom[p]] += gy[p];
+		Gz[basis2Atom[p]] += gz[p];
+	}
+	free(sch); free(map); free(maxL); free(SS); free(PM); free(gx); free(gy); free(gz); free(wt); free(bc);
+}
+
+/* the reference's cut-off GRAD_SCHWARZ_CUTOFF; or_grad_eri_cut with cutoff 0
+ * is the exact derivative of the two-electron energy */
+void or_grad_eri(int childID, int nCPU, int n, const struct GTOBasis_t *g, const int *basis2Atom,
+                 const double *PT, const double *PS, double Kfactor, double *Gx, double *Gy, double *Gz)
+{
+	or_grad_eri_cut(childID, nCPU, n, g, basis2Atom, PT, PS, Kfactor, OR_GRAD_CUTOFF, Gx, Gy, Gz);
+}
+
+/* Hartree-Fock forces F = -dE/dR per atom (src/grad.c:2155-2420 without the
+ * DFT branch): nuclear repulsion, overlap against the energy-weighted
+ * density QT = sum_occ e_i C_i C_i^T, kinetic and attraction (basis-function
+ * and operator derivatives) against PT, the field terms when (Ex, Ey, Ez) is
+ * not zero, and or_grad_eri with exchange weight Kfactor.  CA/CB hold one
+ * orbital per row (C[i*n + p]). */
+void or_uhf_force(int n, const struct GTOBasis_t *g, const struct Molecule_t *mol, int nEA, int nEB,
+                  const double *CA, const double *CB, const double *eA, const double *eB,
+                  double Ex, double Ey, double Ez, double Kfactor, double *Fx, double *Fy, double *Fz)
+{
+	int na = mol->nAtom;
+	int *b2a = malloc(sizeof(int) * n);
+	for (int p = 0; p < n; p++)
+		for (int i = 0; i < na; i++)
+			if (g[p].x0 == mol->x[i] && g[p].y0 == mol->y[i] && g[p].z0 == mol->z[i]) { b2a[p] = i; break; }
+	double *PA = calloc((size_t)n * n, sizeof(double)), *PB = calloc((size_t)n * n, sizeof(double));
+	double *PT = malloc(sizeof(double) * n * n), *PS = malloc(sizeof(double) * n * n);
+	double *QT = calloc((size_t)n * n, sizeof(double));
+	for (int p = 0; p < n; p++)
+		for (int q = 0; q < n; q++) {
+			for (int i = 0; i < nEA; i++) {
+				PA[p * n + q] += CA[i * n + p] * CA[i * n + q];
+				QT[p * n + q] += eA[i] * CA[i * n + p] * CA[i * n + q];
+			}
+			for (int i = 0; i < nEB; i++) {
+				PB[p * n + q] += CB[i * n + p] * CB[i * n + q];
+				QT[p * n + q] += eB[i] * CB[i * n + p] * CB[i * n + q];
+			}
+			PT[p * n + q] = PA[p * n + q] + PB[p * n + q];
+			PS[p * n + q] = PA[p * n + q] - PB[p * n + q];
+		}
+	double *f[3];
+	for (int d = 0; d < 3; d++) f[d] = calloc(na, sizeof(double));
+	or_grad_vnn(mol, f[0], f[1], f[2]);
+	for (int p = 0; p < n; p++)
+		for (int q = 0; q < n; q++)
+			for (int d = 0; d < 3; d++) {
+				f[d][b2a[p]] -= or_grad_overlap(p, q, g, d) * QT[p * n + q];
+				f[d][b2a[q]] -= or_grad_overlap(q, p, g, d) * QT[p * n + q];
+				f[d][b2a[p]] += or_grad_kinetic(p, q, g, d) * PT[p * n + q];
+				f[d][b2a[q]] += or_grad_kinetic(q, p, g, d) * PT[p * n + q];
+				f[d][b2a[p]] += 2.0 * or_grad_nai(p, q, g, mol, d) * PT[p * n + q];
+			}
+	for (int p = 0; p < n; p++)
+		for (int q = 0; q <= p; q++)
+			for (int i = 0; i < na; i++) {
+				double e[3];
+				or_gto_efi(p, q, g, mol->x[i], mol->y[i], mol->z[i], e, e + 1, e + 2);
+				double w = (p != q ? 2.0 : 1.0) * mol->Z[i] * PT[p * n + q];
+				for (int d = 0; d < 3; d++) f[d][i] += e[d] * w;
+			}
+	if (Ex != 0.0 || Ey != 0.0 || Ez != 0.0) {
+		double E3[3] = {Ex, Ey, Ez};
+		for (int p = 0; p < n; p++)
+			for (int q = 0; q < n; q++)
+				for (int c = 0; c < 3; c++)
+					for (int d = 0; d < 3; d++)
+						f[d][b2a[p]] += 2.0 * or_grad_moment(p, q, g, c == 0, c == 1, c == 2, 0.0, 0.0, 0.0, d) *
+						                PT[p * n + q] * E3[c];
+		for (int i = 0; i < na; i++)
+			for (int d = 0; d < 3; d++) f[d][i] -= E3[d] * mol->Z[i];
+	}
+	or_grad_eri(0, 1, n, g, b2a, PT, PS, Kfactor, f[0], f[1], f[2]);
+	for (int i = 0; i < na; i++) { Fx[i] = -f[0][i]; Fy[i] = -f[1][i]; Fz[i] = -f[2][i]; }
+	for (int d = 0; d < 3; d++) free(f[d]);
+	free(b2a); free(PA); free(PB); free(PT); free(PS); free(QT);
+}
+
 /* convenience for tests: free what or_gen_basis / or_read_xyz made */
 void or_free_basis(int n, struct GTOBasis_t *g)
 {

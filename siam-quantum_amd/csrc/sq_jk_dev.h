@@ -45,7 +45,7 @@ struct __attribute__((aligned(16))) PrimPair {
   double cc;         // c_a c_b exp(-alpha_a alpha_b AB^2 / p) / p
   double eb;         //         exp(-alpha_a alpha_b AB^2 / p) / p
   double hp;         // 1 / (2p)
-  double pad;
+  double ea;         // alpha_a (derivatives with respect to the centre of a)
 };
 
 // One bra group x ket group block of a launch.

@@ -36,7 +36,9 @@
 #include <vector>
 
 #include "../../include/sq_jk.h"
+#include "../../include/sq_grad.h"
 #include "sq_jk_classes.h"
+#include "sq_grad_classes.h"
 #include "sq_jk_wd.h"
 #include "sq_one_e.h"
 #include "sq_internal.h"
@@ -98,6 +100,14 @@ ClassLauncher launcher_for(int la, int lb, int lc, int ld) {
   if (la == a && lb == b && lc == c && ld == d) return &sqjk_launch_##a##b##c##d;
   SQJK_CLASSES(SQJK_CASE)
 #undef SQJK_CASE
+  return nullptr;
+}
+
+GradLauncher grad_launcher_for(int la, int lb, int lc, int ld) {
+#define SQGRAD_CASE(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return &sqgrad_launch_##a##b##c##d;
+  SQJK_CLASSES(SQGRAD_CASE)
+#undef SQGRAD_CASE
   return nullptr;
 }
 
@@ -280,7 +290,7 @@ int launch_boys_eval(int L, const double* tab, const double* T, double* F, int n
                        st, tab, T, F, nT);                                                                   \
     return 0;
     SQJK_BOYS_CASE(0) SQJK_BOYS_CASE(1) SQJK_BOYS_CASE(2) SQJK_BOYS_CASE(3) SQJK_BOYS_CASE(4)
-    SQJK_BOYS_CASE(5) SQJK_BOYS_CASE(6) SQJK_BOYS_CASE(7) SQJK_BOYS_CASE(8)
+    SQJK_BOYS_CASE(5) SQJK_BOYS_CASE(6) SQJK_BOYS_CASE(7) SQJK_BOYS_CASE(8) SQJK_BOYS_CASE(9)
 #undef SQJK_BOYS_CASE
     default: return -1;
   }
@@ -534,6 +544,7 @@ void plan_pairs(sqjk_ctx* c, const std::vector<double>& pexp, const std::vector<
           q.eb = K;
           q.cc = pcoef[A.prim_off + ka] * pcoef[B.prim_off + kb] * K;
           q.hp = 0.5 / p;
+          q.ea = ea;
           c->pp.push_back(q);
         }
       c->pairs[pair_class(A.l, B.l)].push_back(pr);
@@ -1607,6 +1618,157 @@ int sqjk_build_device_noalloc(sqjk_ctx* c, const double* dPA, const double* dPB,
   return SQJK_OK;
 }
 
+// Two-electron part of the energy gradient (src/grad.c:1734-2090).  The J/K
+// work plan of the context, made for a Schwarz cut-off of cutoff/16 (the
+// largest density-block sum of a normal density is well below 16), and the
+// exact density-weighted test per quartet in the kernels.
+int sqjk_grad_eri(sqjk_ctx* c, const double* PA, const double* PB, double Kfactor, double cutoff,
+                  const int* basis2Atom, int nAtom, double* G) {
+  if (!c || !PA || !PB || !basis2Atom || !G || nAtom <= 0 || !(cutoff >= 0.0))
+    return fail(SQJK_EARG, "sqjk_grad_eri: invalid arguments");
+  const int n = c->n;
+  for (int p = 0; p < n; ++p)
+    if (basis2Atom[p] < 0 || basis2Atom[p] >= nAtom) return fail(SQJK_EARG, "sqjk_grad_eri: basis2Atom out of range");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nn = (size_t)n * n;
+  HIPCHK(hipMemcpyAsync(c->d_DA, PA, nn * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_DB, PB, nn * 8, hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_density_sum, dim3(grid_for((long long)nn, 256)), dim3(256), 0, c->st, c->d_DA, c->d_DB,
+                     c->d_DJ, nn);
+  hipLaunchKernelGGL(k_pair_max, dim3(grid_for((long long)c->nref * c->nref, 256)), dim3(256), 0, c->st, c->d_DJ,
+                     c->d_DA, c->d_DB, c->d_ref, c->nref, n, c->d_PM);
+  HIPCHK(hipGetLastError());
+  int rc;
+  const double thr = cutoff / 16.0;
+  if ((rc = make_plan(c, thr, thr))) return rc;
+  double* d_g = nullptr;
+  if ((rc = dalloc(&d_g, (size_t)3 * c->nsub))) return rc;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(3LL * c->nsub, 256)), dim3(256), 0, c->st, d_g, (size_t)3 * c->nsub, 0.0);
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (const sqjk_ctx::Launch& L : c->launches) {
+    GradArgs a{};
+    a.c = base_args(c);
+    a.c.bra = c->d_pairs[L.X];
+    a.c.ket = c->d_pairs[L.Y];
+    a.c.seg = c->d_seg;
+    a.c.tile = c->d_tile + L.tile0;
+    a.c.rbra = c->d_rbra;
+    a.c.rlo = c->d_rlo;
+    a.c.rhi = c->d_rhi;
+    a.c.same = (L.X == L.Y);
+    a.c.total = L.ntile;
+    a.c.fixed_cut = cutoff;
+    a.c.npb = (int)c->pairs[L.X].size();
+    a.c.npk = (int)c->pairs[L.Y].size();
+    a.gsub = d_g;
+    a.kfac = Kfactor;
+    GradLauncher f = grad_launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
+    if (!f || f(a, 0, c->st) < 0) {
+      block_put(d_g);
+      return fail(SQJK_EHIP, "gradient kernel launch failed");
+    }
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->st));
+  std::vector<double> gs((size_t)3 * c->nsub);
+  hipError_t e = hipMemcpyAsync(gs.data(), d_g, gs.size() * 8, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  block_put(d_g);
+  if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_grad_eri: ") + hipGetErrorString(e));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last.kernel_ms = ms;
+  for (int k = 0; k < c->nsub; ++k) {
+    const int at = basis2Atom[c->sub[k].bf_off];
+    for (int x = 0; x < 3; ++x) G[3 * at + x] += gs[3 * k + x];
+  }
+  return SQJK_OK;
+}
+
+int sqjk_grad_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, const double* PT,
+                           const double* QT, const int* basis2Atom, int device, double* G) {
+  if (nBasis <= 0 || !gto || !mol || mol->nAtom <= 0 || !PT || !QT || !basis2Atom || !G)
+    return fail(SQJK_EARG, "sqjk_grad_one_electron: invalid arguments");
+  for (int p = 0; p < nBasis; ++p)
+    if (basis2Atom[p] < 0 || basis2Atom[p] >= mol->nAtom)
+      return fail(SQJK_EARG, "sqjk_grad_one_electron: basis2Atom out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(SQJK_ENODEV, "no HIP device is visible; this library has no CPU path");
+  if (device < 0 || device >= ndev) return fail(SQJK_EARG, "sqjk_grad_one_electron: device ordinal out of range");
+  HIPCHK(hipSetDevice(device));
+  sqjk_ctx c;
+  c.n = nBasis;
+  std::vector<double> pexp, pcoef;
+  int rc = plan_shells(&c, nBasis, gto, pexp, pcoef);
+  if (rc) return rc;
+  plan_pairs(&c, pexp, pcoef);
+  const int na = mol->nAtom;
+  std::vector<double> atoms((size_t)4 * na);
+  for (int k = 0; k < na; ++k) {
+    atoms[4 * k + 0] = mol->x[k]; atoms[4 * k + 1] = mol->y[k]; atoms[4 * k + 2] = mol->z[k];
+    atoms[4 * k + 3] = (double)mol->Z[k];
+  }
+  const size_t nn = (size_t)nBasis * nBasis;
+  thread_local hipStream_t st = nullptr;
+  thread_local int st_dev = -1;
+  if (!st || st_dev != device) {
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    st_dev = device;
+  }
+  std::vector<void*> bufs;
+  bool up_ok = true;
+  auto up = [&](const void* h, size_t bytes) -> void* {
+    void* d = nullptr;
+    if (!(d = block_get(std::max<size_t>(bytes, 8), false))) { up_ok = false; return nullptr; }
+    bufs.push_back(d);
+    if (h && bytes && hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) != hipSuccess) up_ok = false;
+    return d;
+  };
+  auto release = [&] {
+    (void)hipStreamSynchronize(st);
+    for (void* d : bufs) block_put(d);
+  };
+  const std::vector<double>& boys = boys_table();
+  std::vector<double> zero((size_t)3 * std::max(c.nsub, na), 0.0);
+  OneEGradArgs a{};
+  a.sh = (const SubShell*)up(c.sub.data(), c.sub.size() * sizeof(SubShell));
+  a.pexp = (const double*)up(pexp.data(), pexp.size() * 8);
+  a.pcoef = (const double*)up(pcoef.data(), pcoef.size() * 8);
+  a.bscale = (const double*)up(c.bscale.data(), c.bscale.size() * 8);
+  a.atom = (const double*)up(atoms.data(), atoms.size() * 8);
+  a.boys = (const double*)up(boys.data(), boys.size() * 8);
+  a.PT = (const double*)up(PT, nn * 8);
+  a.QT = (const double*)up(QT, nn * 8);
+  a.gsub = (double*)up(zero.data(), (size_t)3 * c.nsub * 8);
+  a.gatom = (double*)up(zero.data(), (size_t)3 * na * 8);
+  a.natom = na;
+  a.n = nBasis;
+  if (!up_ok) {
+    release();
+    return fail(SQJK_ENOMEM, "sqjk_grad_one_electron: device allocation or upload failed");
+  }
+  for (int X = 0; X < kNPairClass; ++X) {
+    a.pr = (const PairRec*)up(c.pairs[X].data(), c.pairs[X].size() * sizeof(PairRec));
+    a.np = (int)c.pairs[X].size();
+    if (!up_ok || launch_one_electron_grad(X, a, st) != 0) {
+      release();
+      return fail(SQJK_EHIP, "sqjk_grad_one_electron: kernel launch failed");
+    }
+  }
+  std::vector<double> gs((size_t)3 * c.nsub), ga((size_t)3 * na);
+  hipError_t e = hipMemcpyAsync(gs.data(), a.gsub, gs.size() * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ga.data(), a.gatom, ga.size() * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  release();
+  if (e != hipSuccess) return fail(SQJK_EHIP, std::string("sqjk_grad_one_electron: ") + hipGetErrorString(e));
+  for (int k = 0; k < c.nsub; ++k) {
+    const int at = basis2Atom[c.sub[k].bf_off];
+    for (int x = 0; x < 3; ++x) G[3 * at + x] += gs[3 * k + x];
+  }
+  for (int k = 0; k < 3 * na; ++k) G[k] += ga[k];
+  return SQJK_OK;
+}
+
 int sqjk_sync(sqjk_ctx* c) {
   if (!c) return fail(SQJK_EARG, "sqjk_sync: null context");
   HIPCHK(hipSetDevice(c->device));
@@ -1615,7 +1777,7 @@ int sqjk_sync(sqjk_ctx* c) {
 }
 
 int sqjk_boys(int L, int nT, const double* T, int fp32, int device, double* F) {
-  if (L < 0 || L > 8 || nT < 0 || (nT > 0 && (!T || !F))) return fail(SQJK_EARG, "sqjk_boys: invalid arguments");
+  if (L < 0 || L > 9 || nT < 0 || (nT > 0 && (!T || !F))) return fail(SQJK_EARG, "sqjk_boys: invalid arguments");
   if (nT == 0) return SQJK_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)

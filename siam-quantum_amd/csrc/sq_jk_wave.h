@@ -13,13 +13,13 @@
 
 namespace sqjk {
 
-// (t,u,v) of the compact Hermite index, up to total order 8
+// (t,u,v) of the compact Hermite index, up to total order 9
 struct HermTab {
-  unsigned char t[nherm(8)], u[nherm(8)], v[nherm(8)];
+  unsigned char t[nherm(9)], u[nherm(9)], v[nherm(9)];
 };
 __host__ __device__ constexpr HermTab make_herm_tab() {
   HermTab h{};
-  for (int n = 0; n <= 8; ++n)
+  for (int n = 0; n <= 9; ++n)
     for (int m = 0; m <= n; ++m)
       for (int v = 0; v <= m; ++v) {
         const int u = m - v, t = n - m;

@@ -66,9 +66,9 @@ constexpr double kPrimCutoff = 1.0e-15;            // bare prefactor cut
 // ---------------------------------------------------------------- Boys
 // Table: F_m(k*kBoysStep) for k in [0,kBoysNT), m in [0,kBoysNM), then
 // exp(-k*kBoysStep) in column kBoysNM (row stride kBoysCols).
-constexpr int kBoysNT = 601;
+constexpr int kBoysNT = 651;
 constexpr double kBoysStep = 0.1;
-constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 60
+constexpr double kBoysTMax = (kBoysNT - 1) * kBoysStep;  // 65
 constexpr int kBoysNM = 28;
 constexpr int kBoysCols = kBoysNM + 1;
 constexpr int kBoysTaylor = 9;  // terms of the Taylor expansion
@@ -78,14 +78,14 @@ constexpr int kBoysLdsW = kBoysTaylor + 1;  // LDS row: F_L .. F_{L+8}, exp(-T_k
 // sqrt(pi/T)/2 and the upward recursion without the exp(-T) term are within
 // 2e-17 relative of F_m for every m <= L (checked against mpmath in steps of
 // 0.5), so a class of total angular momentum L needs table rows up to
-// kBoysTCut[L] only.
-constexpr double kBoysTCut[9] = {36.5, 40.5, 44.0, 47.0, 50.0, 52.5, 55.0, 57.5, 60.0};
+// kBoysTCut[L] only.  L = 9 is the (dd|dd) energy gradient.
+constexpr double kBoysTCut[10] = {36.5, 40.5, 44.0, 47.0, 50.0, 52.5, 55.0, 57.5, 60.0, 62.5};
 template <int L>
 constexpr int boys_rows() {
-  static_assert(L >= 0 && L <= 8, "Boys table cut-offs cover L <= 8");
+  static_assert(L >= 0 && L <= 9, "Boys table cut-offs cover L <= 9");
   return (int)(kBoysTCut[L] / kBoysStep + 0.5) + 1;
 }
-static_assert(boys_rows<8>() <= kBoysNT, "table shorter than the largest cut-off");
+static_assert(boys_rows<9>() <= kBoysNT, "table shorter than the largest cut-off");
 
 // LDS copy for a class of total angular momentum L: per grid point the
 // Taylor coefficients F_L..F_{L+8} and exp(-T_k).

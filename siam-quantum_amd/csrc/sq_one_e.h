@@ -23,4 +23,22 @@ struct OneEArgs {
 // pair_class: 0 (s,s) 1 (p,s) 2 (p,p) 3 (d,s) 4 (d,p) 5 (d,d).  0 on success.
 int launch_one_electron(int pair_class, const OneEArgs& a, hipStream_t st);
 
+// One-electron part of the energy gradient (sq_grad_one.hip).
+struct OneEGradArgs {
+  const SubShell* __restrict__ sh;
+  const PairRec* __restrict__ pr;     // all unique sub-shell pairs of one class
+  const double* __restrict__ pexp;
+  const double* __restrict__ pcoef;
+  const double* __restrict__ bscale;
+  const double* __restrict__ atom;    // x, y, z, Z per nucleus
+  const double* __restrict__ boys;
+  const double* __restrict__ PT;      // total density (n x n)
+  const double* __restrict__ QT;      // energy-weighted density (n x n)
+  double* gsub;                       // [nsub][3] gradient on basis-function centres (atomics)
+  double* gatom;                      // [natom][3] gradient on nuclei, operator part (atomics)
+  int np, natom, n;
+};
+
+int launch_one_electron_grad(int pair_class, const OneEGradArgs& a, hipStream_t st);
+
 }  // namespace sqjk

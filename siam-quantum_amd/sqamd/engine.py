@@ -129,6 +129,19 @@ class JKEngine:
     def set_profile(self, on=True):
         _check(lib().sqjk_set_profile(self._h, int(on)), "sqjk_set_profile")
 
+    def grad_eri(self, PA, PB, basis2atom, natom, kfactor=1.0, cutoff=1e-10):
+        """Two-electron energy gradient per atom, [natom, 3] (sqjk_grad_eri)."""
+        L = lib()
+        P32 = C.POINTER(C.c_int)
+        L.sqjk_grad_eri.argtypes = [C.c_void_p, PD, PD, C.c_double, C.c_double, P32, C.c_int, PD]
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = np.ascontiguousarray(PB, dtype=np.float64)
+        b2a = np.ascontiguousarray(basis2atom, dtype=np.int32)
+        G = np.zeros((natom, 3))
+        _check(L.sqjk_grad_eri(self._h, dptr(PA), dptr(PB), float(kfactor), float(cutoff),
+                               b2a.ctypes.data_as(P32), int(natom), dptr(G)), "sqjk_grad_eri")
+        return G
+
     def class_times(self):
         """Kernel ms per class of the last build (profile mode), class_stats order."""
         t = (C.c_double * 21)()
@@ -150,3 +163,42 @@ class JKEngine:
             self.close()
         except Exception:
             pass
+
+
+def grad_one_electron(bas, mol, PT, QT, basis2atom, device=0):
+    """One-electron energy gradient per atom, [natom, 3] (sqjk_grad_one_electron)."""
+    from .abi import Molecule
+    L = lib()
+    P32 = C.POINTER(C.c_int)
+    L.sqjk_grad_one_electron.argtypes = [C.c_int, PG, C.POINTER(Molecule), PD, PD, P32, C.c_int, PD]
+    PT = np.ascontiguousarray(PT, dtype=np.float64)
+    QT = np.ascontiguousarray(QT, dtype=np.float64)
+    b2a = np.ascontiguousarray(basis2atom, dtype=np.int32)
+    na = mol.ptr.contents.nAtom
+    G = np.zeros((na, 3))
+    _check(L.sqjk_grad_one_electron(bas.n, bas.ptr, mol.ptr, dptr(PT), dptr(QT), b2a.ctypes.data_as(P32),
+                                    int(device), dptr(G)), "sqjk_grad_one_electron")
+    return G
+
+
+def basis_to_atom(mol, bas):
+    """Atom of each basis function (the first atom at its centre)."""
+    m, g = mol.ptr.contents, bas.ptr
+    out = np.empty(bas.n, dtype=np.int32)
+    for p in range(bas.n):
+        out[p] = next(i for i in range(m.nAtom) if (g[p].x0, g[p].y0, g[p].z0) == (m.x[i], m.y[i], m.z[i]))
+    return out
+
+
+def uhf_force(bas, mol, nEA, nEB, CA, CB, eA, eB, opt):
+    """The uhf_force drop-in: forces -dE/dR per atom, [natom, 3]."""
+    from .abi import Molecule
+    L = lib()
+    L.uhf_force.argtypes = [C.c_int, PG, C.POINTER(Molecule), C.c_int, C.c_int, PD, PD, PD, PD,
+                            C.POINTER(Option), PD, PD, PD]
+    CA, CB, eA, eB = (np.ascontiguousarray(a, dtype=np.float64) for a in (CA, CB, eA, eB))
+    na = mol.ptr.contents.nAtom
+    F = np.zeros((3, na))
+    L.uhf_force(bas.n, bas.ptr, mol.ptr, nEA, nEB, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt),
+                dptr(F[0]), dptr(F[1]), dptr(F[2]))
+    return F.T.copy()

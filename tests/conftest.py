@@ -21,6 +21,16 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def reference_grad():
+    """The reference's force routines (src/grad.c) compiled into oracle/_ref (if built)."""
+    from oracle import ReferenceGrad
+    try:
+        return ReferenceGrad()
+    except FileNotFoundError:
+        pytest.skip("oracle/_ref/libsqref_grad.so not built")
+
+
+@pytest.fixture(scope="session")
 def reference():
     """The reference's own hot-path code compiled into oracle/_ref (if built)."""
     from oracle import Reference

@@ -1,4 +1,4 @@
-"""Writes tests/golden/boys_mpmath.json: F_m(T) for m = 0..8 by mpmath at 40
+"""Writes tests/golden/boys_mpmath.json: F_m(T) for m = 0..9 by mpmath at 40
 significant digits, F_m(T) = T^(-m-1/2) gamma(m+1/2, T) / 2 (lower incomplete
 gamma), F_m(0) = 1/(2m+1).  The T grid covers both branches of the device
 evaluation: the table range with points on and between grid nodes, points
@@ -14,7 +14,7 @@ import mpmath as mp
 mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
 # per-L switch points of the device evaluation (table rows up to these T)
-SWITCH = [36.5, 40.5, 44.0, 47.0, 50.0, 52.5, 55.0, 57.5, 60.0]
+SWITCH = [36.5, 40.5, 44.0, 47.0, 50.0, 52.5, 55.0, 57.5, 60.0, 62.5]
 
 
 def boys(m, T):
@@ -42,10 +42,10 @@ def grid():
 
 def main():
     T = grid()
-    F = [[float(boys(m, t)) for m in range(9)] for t in T]
+    F = [[float(boys(m, t)) for m in range(10)] for t in T]
     out = os.path.join(HERE, "boys_mpmath.json")
     with open(out, "w") as f:
-        json.dump({"T": T, "F": F, "mmax": 8, "digits": 40}, f, indent=0)
+        json.dump({"T": T, "F": F, "mmax": 9, "digits": 40}, f, indent=0)
     print(f"{len(T)} points -> {out}")
 
 

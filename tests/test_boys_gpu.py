@@ -1,7 +1,7 @@
 """Device Boys function against mpmath (tests/golden/boys_mpmath.json).
 
 sqjk_boys runs the integral kernels' own Boys code (boys<L, R> in
-csrc/sq_md.h) on the device for every total angular momentum L <= 8, on a T
+csrc/sq_md.h) on the device for every total angular momentum L <= 9, on a T
 grid that straddles each L's switch from the LDS table (Taylor expansion plus
 downward recursion) to the large-T branch (sqrt(pi/T)/2 plus upward
 recursion without exp(-T)), up to T = 80.  Both the FP64 path and the FP32
@@ -32,7 +32,7 @@ def _device_boys(L, T, fp32):
 
 
 @pytest.mark.parametrize("fp32,tol", [(False, 2e-14), (True, 2e-6)])
-@pytest.mark.parametrize("L", list(range(9)))
+@pytest.mark.parametrize("L", list(range(10)))
 def test_device_boys_vs_mpmath(L, fp32, tol):
     fx = json.load(open(FIX))
     T = np.array(fx["T"])
@@ -49,4 +49,4 @@ def test_device_boys_rejects_bad_input():
     F = np.zeros(2)
     lib().sqjk_boys.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_double)]
     assert lib().sqjk_boys(1, 1, dptr(T), 0, 0, dptr(F)) != 0
-    assert lib().sqjk_boys(9, 1, dptr(np.zeros(1)), 0, 0, dptr(F)) != 0
+    assert lib().sqjk_boys(10, 1, dptr(np.zeros(1)), 0, 0, dptr(F)) != 0
