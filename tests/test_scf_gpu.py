@@ -47,6 +47,24 @@ def test_water_631gs_vs_gamess():
     assert abs(E - gms) < 2e-8
 
 
+def test_water16_energy_vs_reference_fixture():
+    """(H2O)16 RHF/6-31G* (304 functions): the device SCF, one-step cut-offs
+    and SCFConv 1e-10, against the converged energy on the reference's own
+    J/K (tests/golden/scf_w16.json, make_scf_fixture.py), to 1e-8 Eh.  The
+    reference-default run (3-step cut-offs, SCFConv 1e-8) of the bench lands
+    within the same bar."""
+    from molecules import cluster
+    fx = json.load(open(os.path.join(HERE, "golden", "scf_w16.json")))
+    mol, bas = cluster(16, 1601)
+    assert bas.n == fx["n"]
+    ne = mol.n_electrons
+    E = _uhf(mol, bas, ne // 2, ne // 2, SCFConv=1e-10, SCFAccuracy=1)
+    E3 = _uhf(mol, bas, ne // 2, ne // 2, SCFConv=1e-8)
+    print(f"E(gpu, 1-step) = {E:.10f}  E(gpu, 3-step) = {E3:.10f}  reference J/K = {fx['energy']:.10f}")
+    assert abs(E - fx["energy"]) < 1e-8
+    assert abs(E3 - fx["energy"]) < 1e-8
+
+
 def test_reference_defaults_three_step(oracle):
     """Reference defaults (3-step cut-offs 1e-3/1e-6/1e-9, SCFConv 1e-8,
     DIIS4, incremental builds).  Parity: the oracle's SCF on the same
