@@ -328,61 +328,133 @@ def batch_instances(count=256, seed=4256, uhf_fraction=0.25, sigma=0.05):
 
 
 # ------------------------------------------------------- served requests
-def random_chain(natom, rng, bond=(1.0, 1.5), h_bond=(1.0, 1.1), min_heavy=2.0, min_any=1.5):
-    """A self-avoiding chain of heavy atoms (C 70 %, N 15 %, O 15 %) with
-    bond lengths U(bond) angstrom, hydrogens on the chain atoms at U(h_bond)
-    (at most three per atom); non-bonded heavy atoms at least `min_heavy`
-    apart and every other non-bonded pair at least `min_any`.  About 60 % of
-    the atoms are heavy.  Returns (Z, xyz angstrom)."""
-    nheavy = max(1, int(round(0.6 * natom)))
-    nh = natom - nheavy
-    for _attempt in range(200):
-        Z = [int(rng.choice([6, 7, 8], p=[0.7, 0.15, 0.15])) for _ in range(nheavy)]
-        X = [np.zeros(3)]
-        prev_dir = None
+VALENCE = {6: 4, 7: 3, 8: 2}
+
+
+def _h_directions(bonds, nh, rng):
+    """Unit vectors of nh hydrogens on an atom whose other bonds point along
+    `bonds` (unit vectors): near-tetrahedral positions in a frame fixed by
+    the bonds (a random azimuth when the frame is not fixed)."""
+    if nh == 0:
+        return []
+    tet = np.deg2rad(109.47)
+    def perp(u):
+        v = rng.standard_normal(3)
+        v -= np.dot(v, u) * u
+        return v / np.linalg.norm(v)
+    if not bonds:  # lone atom: a tetrahedron
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        return ([u] + _h_directions([u], nh - 1, rng))[:nh]
+    if len(bonds) == 1:  # three positions at the tetrahedral angle around the bond
+        u = bonds[0]
+        e1 = perp(u)
+        e2 = np.cross(u, e1)
+        phi = rng.uniform(0, 2 * np.pi)
+        out = [np.cos(tet) * u + np.sin(tet) * (np.cos(phi + 2 * np.pi * k / 3) * e1 +
+                                                 np.sin(phi + 2 * np.pi * k / 3) * e2) for k in range(3)]
+        return out[:nh]
+    # two bonds: along the outer bisector (one H) or above and below the bond plane (two H)
+    u1, u2 = bonds
+    b = -(u1 + u2)
+    b = b / np.linalg.norm(b) if np.linalg.norm(b) > 1e-6 else perp(u1)
+    n = np.cross(u1, u2)
+    n = n / np.linalg.norm(n) if np.linalg.norm(n) > 1e-6 else perp(b)
+    if nh == 1:
+        return [b]
+    half = tet / 2
+    return [np.cos(half) * b + np.sin(half) * n, np.cos(half) * b - np.sin(half) * n][:nh]
+
+
+BOND = {(6, 6): 1.53, (6, 7): 1.47, (6, 8): 1.43}
+H_BOND = {6: 1.09, 7: 1.01, 8: 0.96}
+
+
+def random_chain(natom, rng, min_heavy=2.3, min_any=1.6, radical=False):
+    """A saturated open chain of about `natom` atoms, built like an alkane
+    with ether and amine links: heavy atoms C (70 %), N (15 %), O (15 %), no
+    two heteroatoms bonded; bond lengths C-C 1.53, C-N 1.47, C-O 1.43
+    angstrom (+-0.03); bond angles 109.5 +- 4 degrees; dihedrals anti (60 %)
+    or gauche (+-60), +-10 degrees; every heavy atom completed with
+    hydrogens to its valence (C 4, N 3, O 2) on near-tetrahedral positions
+    (C-H 1.09, N-H 1.01, O-H 0.96 +- 0.02).  Non-bonded heavy atoms stay
+    `min_heavy` apart, every other non-bonded pair `min_any`.  With
+    `radical`, one hydrogen is left out (an odd electron count).  Returns
+    (Z, xyz angstrom)."""
+    # a saturated chain carries about 1.55 hydrogens per heavy atom
+    nheavy = max(2, int(round(natom / 2.55)))
+    theta = np.deg2rad(180.0 - 109.5)
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    def hydrogens(a, X, H, nbr, Z):
+        # the hydrogens of heavy atom a, or None if one comes too close to another atom
+        out = []
+        for d in _h_directions([unit(q - X[a]) for q in nbr], max(0, VALENCE[Z[a]] - len(nbr)), rng):
+            p = X[a] + d * (H_BOND[Z[a]] + rng.uniform(-0.02, 0.02))
+            if not (all(np.linalg.norm(p - q) >= min_any for i, q in enumerate(X) if i != a) and
+                    all(np.linalg.norm(p - q) >= min_any for q in H + out)):
+                return None
+            out.append(p)
+        return out
+
+    def bond(z1, z2):
+        return BOND[(min(z1, z2), max(z1, z2))] + rng.uniform(-0.03, 0.03)
+
+    for _attempt in range(50):
+        Z = []
+        for k in range(nheavy):
+            z = int(rng.choice([6, 7, 8], p=[0.7, 0.15, 0.15]))
+            Z.append(6 if (k > 0 and Z[-1] != 6 and z != 6) else z)
+        X, H = [np.zeros(3)], []
         ok = True
+        # grow the chain; atom k-1's hydrogens are placed once atom k fixes its second bond
         for k in range(1, nheavy):
-            for _try in range(200):
-                d = rng.standard_normal(3)
-                d /= np.linalg.norm(d)
-                if prev_dir is not None and np.dot(d, prev_dir) < 0.0:  # keep the chain from folding back
+            for _try in range(100):
+                if k == 1:
+                    d = unit(rng.standard_normal(3))
+                else:
+                    u = unit(X[k - 1] - X[k - 2])
+                    ref = X[k - 3] - X[k - 2] if k >= 3 else rng.standard_normal(3)
+                    e1 = ref - np.dot(ref, u) * u
+                    e1 = unit(e1) if np.linalg.norm(e1) > 1e-6 else unit(np.cross(u, rng.standard_normal(3)))
+                    e2 = np.cross(u, e1)
+                    phi = np.deg2rad(rng.choice([180.0, 60.0, -60.0], p=[0.6, 0.2, 0.2]) + rng.uniform(-10, 10))
+                    th = theta + np.deg2rad(rng.uniform(-4, 4))
+                    d = np.cos(th) * u + np.sin(th) * (np.cos(phi) * e1 + np.sin(phi) * e2)
+                p = X[-1] + d * bond(Z[k - 1], Z[k])
+                if not (all(np.linalg.norm(p - q) >= min_heavy for q in X[:-1]) and
+                        all(np.linalg.norm(p - q) >= min_any for q in H)):
                     continue
-                p = X[-1] + d * rng.uniform(*bond)
-                if all(np.linalg.norm(p - q) >= min_heavy for q in X[:-1]):
+                nbr = ([X[k - 2]] if k >= 2 else []) + [p]
+                hs = hydrogens(k - 1, X + [p], H, nbr, Z)
+                if hs is not None:
                     X.append(p)
-                    prev_dir = d
+                    H += hs
                     break
             else:
                 ok = False
                 break
         if not ok:
             continue
-        count = [0] * nheavy
-        H = []
-        for _ in range(nh):
-            for _try in range(500):
-                a = int(rng.integers(nheavy))
-                if count[a] >= 3:
-                    continue
-                d = rng.standard_normal(3)
-                d /= np.linalg.norm(d)
-                p = X[a] + d * rng.uniform(*h_bond)
-                if all(np.linalg.norm(p - q) >= min_any for i, q in enumerate(X) if i != a) and \
-                        all(np.linalg.norm(p - q) >= min_any for q in H):
-                    H.append(p)
-                    count[a] += 1
-                    break
-            else:
-                ok = False
+        for _try in range(50):  # the last atom's hydrogens: a random azimuth about its bond
+            hs = hydrogens(nheavy - 1, X, H, [X[-2]], Z)
+            if hs is not None:
+                H += hs
                 break
-        if ok:
-            return Z + [1] * nh, np.array(X + H)
+        else:
+            continue
+        if radical and H:
+            H.pop(int(rng.integers(len(H))))
+        return Z + [1] * len(H), np.array(X + H)
     raise RuntimeError("random_chain: placement failed")
 
 
-def serving_requests(count=16, rate=4.0, seed=5005, steps=2, atoms=(10, 200), sigma=0.05):
+def serving_requests(count=16, rate=4.0, seed=5005, steps=2, atoms=(10, 200), sigma=0.05, radicals=0.25):
     """Seeded requests of the serving workload: atom counts log-uniform in
-    `atoms`, random C/H/N/O chains (random_chain), `steps` geometry steps
+    `atoms`, saturated C/H/N/O chains (random_chain), a fraction
+    `radicals` of them with one hydrogen left out, `steps` geometry steps
     each displaced by N(0, sigma) angstrom from the base geometry, Poisson
     arrivals at `rate` per second.  RHF for an even electron count, else a
     UHF doublet.  Returns dicts with Z, xyz_bohr [steps, natom, 3], charge,
@@ -392,9 +464,9 @@ def serving_requests(count=16, rate=4.0, seed=5005, steps=2, atoms=(10, 200), si
     for i in range(count):
         t += rng.exponential(1.0 / rate) if i > 0 else 0.0
         natom = int(round(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1])))))
-        Z, X = random_chain(natom, rng)
+        Z, X = random_chain(natom, rng, radical=bool(rng.uniform() < radicals))
         geo = np.stack([(X + rng.normal(0.0, sigma, X.shape)) * ANGSTROM2BOHR for _ in range(steps)])
         ne = int(np.sum(Z))
         out.append({"Z": np.array(Z, dtype=np.int32), "xyz_bohr": geo, "charge": 0,
-                    "multiplicity": 1 if ne % 2 == 0 else 2, "arrival": t, "natom": natom})
+                    "multiplicity": 1 if ne % 2 == 0 else 2, "arrival": t, "natom": len(Z)})
     return out
