@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--forces-only", action="store_true",
+                    help="only the C60 SCF and its forces (a JSON line with the forces timings), for profiling")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -336,6 +338,11 @@ def main():
     print(f"bench: C60 SCF for the step density (n={n})", file=sys.stderr, flush=True)
     orbs = {}
     E_scf, P, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-6)
+    if a.forces_only:
+        eng = JKEngine(bas.ptr, n)
+        print(json.dumps({"forces": forces_bench(eng, mol, bas, P, orbs, a.steps), "scf_energy": E_scf}))
+        eng.close()
+        return
     print(f"bench: C60 E = {E_scf:.8f} Eh in {t_scf:.1f} s; timing builds", file=sys.stderr, flush=True)
     eng = JKEngine(bas.ptr, n)
     eng.set_profile(True)
