@@ -2,6 +2,8 @@
 // angular class, selected at compile time with -DSQ_LA=.. -DSQ_LB=..
 // -DSQ_LC=.. -DSQ_LD=.. (the build compiles this file once per class).
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 
 #include "sq_grad_classes.h"
 
@@ -19,6 +21,7 @@ constexpr int LA = SQ_LA, LB = SQ_LB, LC = SQ_LC, LD = SQ_LD;
 using SI = GradSlice<LA, LB, LC, LD>;
 
 constexpr bool kWave = grad_wave<LA, LB, LC, LD>();
+constexpr bool kBoth = grad_both<LA, LB, LC, LD>();
 
 // workgroups the current device holds at once (occupancy x CUs)
 template <class K>
@@ -58,4 +61,16 @@ int launch_grad(const GradArgs& a, int grid, hipStream_t st) {
 }
 }  // namespace
 
-int SQ_GRAD_LAUNCHER(const sqjk::GradArgs& a, int grid, hipStream_t st) { return launch_grad<kWave>(a, grid, st); }
+int SQ_GRAD_LAUNCHER(const sqjk::GradArgs& a, int grid, hipStream_t st) {
+  if constexpr (kBoth) {
+    static const bool thread = [] {
+      const char* e = std::getenv("SQJK_GRAD_KERNEL");
+      if (e && std::string(e) == "thread") return true;
+      if (e && std::string(e) == "wave") return false;
+      return grad_thread_default<LA, LB, LC, LD>();
+    }();
+    return thread ? launch_grad<false>(a, grid, st) : launch_grad<true>(a, grid, st);
+  } else {
+    return launch_grad<kWave>(a, grid, st);
+  }
+}

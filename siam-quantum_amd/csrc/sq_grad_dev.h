@@ -32,12 +32,13 @@ struct GradArgs {
 };
 
 // Ket-component slices of the gradient kernel: a lane holds the weights of
-// all NAB bra components for SZ ket components (at most 18).
+// all NAB bra components for SZ ket components (SZ = 1 for bras of six or
+// more components; at most nine weights for the small bras).
 template <int LA, int LB, int LC, int LD>
 struct GradSlice {
   static constexpr int NAB = ncart(LA) * ncart(LB);
   static constexpr int NCD = ncart(LC) * ncart(LD);
-  static constexpr int PER = 1;
+  static constexpr int PER = (9 / NAB) > 0 ? (9 / NAB) : 1;
   static constexpr int NSLICE = (NCD + PER - 1) / PER;
   static constexpr int SZ = (NCD + NSLICE - 1) / NSLICE;
 };
@@ -662,12 +663,24 @@ constexpr int grad_block() {
   return 256;
 }
 
-// Classes of total angular momentum 5 and above take the wavefront kernel:
+// Classes of total angular momentum 7 and above take the wavefront kernel:
 // the per-thread kernel's registers (R up to L+1, both pairs' raised Hermite
-// tables) no longer fit.
+// tables) no longer fit.  Classes of L = 5 and 6 have both kernels, chosen
+// at run time (SQJK_GRAD_KERNEL=thread or wave; by default the per-thread
+// kernel, which measures 2-4x faster there despite its register spills: the
+// wavefront kernel keeps only a few lanes busy for classes with few ket
+// components).
 template <int LA, int LB, int LC, int LD>
 constexpr bool grad_wave() {
-  return LA + LB + LC + LD >= 5;
+  return LA + LB + LC + LD >= 7;
+}
+template <int LA, int LB, int LC, int LD>
+constexpr bool grad_both() {
+  return LA + LB + LC + LD == 5 || LA + LB + LC + LD == 6;
+}
+template <int LA, int LB, int LC, int LD>
+constexpr bool grad_thread_default() {
+  return LA + LB + LC + LD <= 6;
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -50,9 +50,10 @@ def test_water_631gs_vs_gamess():
 def test_water16_energy_vs_reference_fixture():
     """(H2O)16 RHF/6-31G* (304 functions): the device SCF, one-step cut-offs
     and SCFConv 1e-10, against the converged energy on the reference's own
-    J/K (tests/golden/scf_w16.json, make_scf_fixture.py), to 1e-8 Eh.  The
-    reference-default run (3-step cut-offs, SCFConv 1e-8) of the bench lands
-    within the same bar."""
+    J/K (tests/golden/scf_w16.json, make_scf_fixture.py), to 1e-8 Eh.  A
+    reference-default run (3-step cut-offs, SCFConv 1e-8: a test on the
+    energy change between iterations, not on the energy's error) is held to
+    1e-6 Eh, the bar of the reference's own logs."""
     from molecules import cluster
     fx = json.load(open(os.path.join(HERE, "golden", "scf_w16.json")))
     mol, bas = cluster(16, 1601)
@@ -62,7 +63,7 @@ def test_water16_energy_vs_reference_fixture():
     E3 = _uhf(mol, bas, ne // 2, ne // 2, SCFConv=1e-8)
     print(f"E(gpu, 1-step) = {E:.10f}  E(gpu, 3-step) = {E3:.10f}  reference J/K = {fx['energy']:.10f}")
     assert abs(E - fx["energy"]) < 1e-8
-    assert abs(E3 - fx["energy"]) < 1e-8
+    assert abs(E3 - fx["energy"]) < 1e-6
 
 
 def test_reference_defaults_three_step(oracle):
