@@ -193,14 +193,35 @@ __host__ __device__ __forceinline__ View make_view(const ClassArgs& A, int inst)
 
 using namespace sqmd;
 
+// Integrals a lane holds (see SliceInfo) and the wavefronts per SIMD the
+// kernel is compiled for, per class.  The default (36 integrals, no bound) has
+// (dp|ps), (ds|ps) and (pp|pp) need a few registers more than 256 plus
+// accumulation registers, one wavefront per SIMD; with 18 integrals ((dp|ps),
+// (pp|pp): one ket component less per slice) or a bound of two wavefronts
+// they run two wavefronts per SIMD without spills ((ds|ps): 8 bytes), and the
+// second wavefront pays for the repeated R and Hermite work of the smaller
+// slices ((dp|ps) 26.8 -> 25.2 ms at C60).  (dp|pp) at 18 measured 5 % slower.
+template <int LA, int LB, int LC, int LD>
+constexpr int slice_budget() {
+  return ((LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 1 && LB == 1 && LC == 1 && LD == 1)) ? 18 : 36;
+}
+template <int LA, int LB, int LC, int LD>
+constexpr int class_min_waves() {
+  return ((LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 1 && LB == 1 && LC == 1 && LD == 1) ||
+          (LA == 2 && LB == 0 && LC == 1 && LD == 0))
+             ? 2
+             : 1;
+}
+
 // Ket-component slices: a lane holds all NAB bra components for SZ ket
-// components (at most 36 integrals, at least one ket component).  Slicing the
-// ket side keeps the W intermediates of different slices disjoint.
+// components (at most slice_budget integrals, at least one ket component).
+// Slicing the ket side keeps the W intermediates of different slices
+// disjoint.
 template <int LA, int LB, int LC, int LD>
 struct SliceInfo {
   static constexpr int NAB = ncart(LA) * ncart(LB);
   static constexpr int NCD = ncart(LC) * ncart(LD);
-  static constexpr int PER = (36 / NAB) > 0 ? (36 / NAB) : 1;
+  static constexpr int PER = (slice_budget<LA, LB, LC, LD>() / NAB) > 0 ? (slice_budget<LA, LB, LC, LD>() / NAB) : 1;
   static constexpr int NSLICE = (NCD + PER - 1) / PER;
   static constexpr int SZ = (NCD + NSLICE - 1) / NSLICE;
 };
@@ -560,7 +581,7 @@ __host__ __device__ __forceinline__ bool density_screen_pass(const View& V, cons
 // BS: workgroup size; the LDS Boys table is shared by the BS/64 wavefronts of
 // a workgroup, so classes with few registers per lane use large workgroups.
 template <int LA, int LB, int LC, int LD, int MODE, typename R = double, int BS = 256>
-__global__ void __launch_bounds__(BS) k_class(ClassArgs A) {
+__global__ void __launch_bounds__(BS, (class_min_waves<LA, LB, LC, LD>())) k_class(ClassArgs A) {
   using SI = SliceInfo<LA, LB, LC, LD>;
   constexpr int L = LA + LB + LC + LD;
   extern __shared__ double lds[];

@@ -88,15 +88,15 @@ static double grad_moment(int i, int j, const struct GTOBasis_t *g, int mx, int 
 	return -(down * lo) + up + up;
 }
 
-/* the library-held context of the two-electron gradient drop-in, rebuilt
- * when the basis pointer or size changes */
+/* the library-held context of the two-electron gradient drop-in: moved to
+ * the new centres when only they changed, rebuilt otherwise */
 static sqjk_ctx *g_ctx;
 static const struct GTOBasis_t *g_ctx_gto;
 static int g_ctx_n;
 
 static sqjk_ctx *grad_context(int nBasis, const struct GTOBasis_t *gto, const char *who)
 {
-	if (g_ctx && (g_ctx_gto != gto || g_ctx_n != nBasis)) {
+	if (g_ctx && (g_ctx_gto != gto || g_ctx_n != nBasis || sqjk_update_geometry(g_ctx, nBasis, gto) != SQJK_OK)) {
 		sqjk_destroy(g_ctx);
 		g_ctx = NULL;
 	}
@@ -105,9 +105,6 @@ static sqjk_ctx *grad_context(int nBasis, const struct GTOBasis_t *gto, const ch
 		if (rc) die_jk(who, rc);
 		g_ctx_gto = gto;
 		g_ctx_n = nBasis;
-	} else {
-		int rc = sqjk_update_geometry(g_ctx, nBasis, gto);  /* the centres may have moved */
-		if (rc) die_jk(who, rc);
 	}
 	return g_ctx;
 }

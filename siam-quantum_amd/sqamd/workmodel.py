@@ -97,7 +97,8 @@ def class_block(la, lb, lc, ld):
     csrc/sq_jk_dev.h): 512 lanes when a lane holds at most 9 integrals of its
     ket-component slice, else 256."""
     nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
-    per = max(36 // nab, 1)
+    budget = 18 if (la, lb, lc, ld) in ((2, 1, 1, 0), (1, 1, 1, 1)) else 36  # slice_budget in csrc/sq_jk_dev.h
+    per = max(budget // nab, 1)
     nslice = (ncd + per - 1) // per
     sz = (ncd + nslice - 1) // nslice
     return 512 if nab * sz <= 9 else 256
