@@ -340,6 +340,11 @@ def main():
     E_scf, P, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-6)
     if a.forces_only:
         eng = JKEngine(bas.ptr, n)
+        from sqamd import tune
+        tpath = os.path.join(tune.DATA, "c60.json")
+        if os.path.exists(tpath) and not a.no_tuning:
+            t = tune.load(tpath)
+            eng.set_tuning(t["rows"], t["split"], t["grid"])
         print(json.dumps({"forces": forces_bench(eng, mol, bas, P, orbs, a.steps), "scf_energy": E_scf}))
         eng.close()
         return
