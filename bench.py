@@ -79,6 +79,50 @@ def scf_once(mol, bas, orbitals=None, **kw):
     return E, occ.T @ occ, dt
 
 
+def mp2_bench(name, steps):
+    """Frozen-core RHF MP2 energy on the device (sqjk_mp2_rhf, Schwarz cut-off
+    1e-10) on the orbitals of a converged SCF (uhf() drop-in, SCFConv 1e-8):
+    (H2O)16/6-31G*, or C60/3-21G, the molecule of the reference's published
+    MP2 timing (47 min on two i7 cores, docs/FAQ.md:121-129)."""
+    from sqamd import JKEngine, inputs
+    from sqamd.engine import frozen_core
+    if name == "c60_321g":
+        Z, X = inputs.c60()
+        mol = inputs.mol_from_angstrom(Z, X)
+        bas = inputs.make_basis(mol, "321g")
+        desc = "C60 truncated icosahedron (C-C 1.45/1.40 A) RHF/3-21G"
+    else:
+        mol, _, d = molecule(name)
+        bas = inputs.make_basis(mol, "631g_star")
+        desc = f"{d} RHF/6-31G*"
+    orbs = {}
+    E, _, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-8)
+    nocc, ncore = mol.n_electrons // 2, frozen_core(mol)
+    eng = JKEngine(bas.ptr, bas.n)
+    runs = []
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        try:
+            r = eng.mp2_rhf(nocc, ncore, orbs["eA"], orbs["CA"], 1e-10)
+        except RuntimeError as e:
+            eng.close()
+            return {"workload": desc, "error": str(e)}
+        r["seconds"] = time.perf_counter() - t0
+        runs.append(r)
+    eng.close()
+    r = min(runs, key=lambda x: x["seconds"])
+    out = {"workload": f"{desc} frozen-core MP2 on the converged SCF, FP64, Schwarz 1e-10",
+           "basis_functions": bas.n, "correlated_occupied": nocc - ncore, "virtual": bas.n - nocc,
+           "scf_energy": E, "scf_seconds": t_scf, "mp2_energy": r["e2"], "seconds": r["seconds"],
+           "runs": len(runs), "integral_store_ms": r["store_ms"], "first_half_ms": r["half1_ms"],
+           "second_half_ms": r["half2_ms"], "second_half_tflops": r["half2_flop"] / (r["half2_ms"] * 1e-3) / 1e12,
+           "first_half_tflops": r["half1_flop"] / (r["half1_ms"] * 1e-3) / 1e12,
+           "quartets": r["quartets"], "device_gb": r["device_bytes"] / 1e9}
+    if name == "c60_321g":
+        out["reference_published"] = "47 min, two i7 @3.4 GHz cores, 6 GB (docs/FAQ.md:121-129)"
+    return out
+
+
 def forces_bench(eng, mol, bas, P, orbs, steps):
     """Hartree-Fock forces of the converged C60 SCF: the two-electron
     gradient (sqjk_grad_eri on the build context, cut-off 1e-10 as the
@@ -313,6 +357,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--forces-only", action="store_true",
                     help="only the C60 SCF and its forces (a JSON line with the forces timings), for profiling")
+    ap.add_argument("--mp2-only", choices=["w16", "c60_321g"], default=None,
+                    help="only the MP2 benchmark of this workload")
+    ap.add_argument("--no-mp2-c60", action="store_true", help="leave out the C60/3-21G MP2 extra")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times), to calibrate "
                          "the rocprofv3 FETCH_SIZE/WRITE_SIZE counters, then exit")
@@ -333,6 +380,9 @@ def main():
         sys.exit(2)
 
     from sqamd import JKEngine, workmodel
+    if a.mp2_only:
+        print(json.dumps({"mp2": mp2_bench(a.mp2_only, a.steps)}))
+        return
     mol, bas, desc = molecule("c60")
     n = bas.n
     print(f"bench: C60 SCF for the step density (n={n})", file=sys.stderr, flush=True)
@@ -458,6 +508,11 @@ def main():
         if a.scf_solves > 0:
             print("bench: SCF solves", file=sys.stderr, flush=True)
             out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)
+        print("bench: MP2 (H2O)16/6-31G*", file=sys.stderr, flush=True)
+        out["mp2"] = mp2_bench("w16", 2)
+        if not a.no_mp2_c60:
+            print("bench: MP2 C60/3-21G", file=sys.stderr, flush=True)
+            out["mp2_c60"] = mp2_bench("c60_321g", 1)
     if not a.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         try:

@@ -81,6 +81,11 @@ class Oracle:
         L.or_grad_eri.argtypes = [I, I, I, PG, C.POINTER(I), PD, PD, D, PD, PD, PD]
         L.or_grad_eri_cut.argtypes = [I, I, I, PG, C.POINTER(I), PD, PD, D, D, PD, PD, PD]
         L.or_uhf_force.argtypes = [I, PG, PM, I, I, PD, PD, PD, PD, D, D, D, D, PD, PD, PD]
+        L.or_mp2_frozen_core.argtypes = [PM]
+        L.or_mp2_frozen_core.restype = I
+        L.or_eri_pairs.argtypes = [I, PG, PD]
+        L.or_mp2_rhf.argtypes = [I, I, I, PD, PD, PG, PD, D]
+        L.or_mp2_rhf.restype = D
 
     # -- scalar pieces -------------------------------------------------
     def boys(self, mmax, T):
@@ -200,6 +205,28 @@ class Oracle:
         self.L.or_uhf_force(n, g, mol, nEA, nEB, dptr(CA), dptr(CB), dptr(eA), dptr(eB), *field, kfactor,
                             dptr(F[0]), dptr(F[1]), dptr(F[2]))
         return F.T.copy()
+
+    # -- MP2 -------------------------------------------------------------
+    def mp2_frozen_core(self, mol):
+        return self.L.or_mp2_frozen_core(mol)
+
+    def eri_pairs(self, g, n):
+        """Dense pair matrix V[pair(p,q), pair(r,s)] = (pq|rs), every integral."""
+        m = n * (n + 1) // 2
+        V = np.zeros((m, m))
+        self.L.or_eri_pairs(n, g, dptr(V))
+        return V
+
+    def mp2_rhf(self, g, n, nocc, ncore, e, Cm, cut=0.0, sch=None):
+        """Closed-shell MP2 energy by the reference's aqij algorithm (all
+        correlated orbitals in one cycle); cut = 0 is the exact energy, 1e-8
+        the reference's MP2_SCHWARZ_CUTOFF rules.  Cm: orbital-major C[k*n+i]."""
+        if sch is None:
+            sch = self.schwarz(g, n)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        sch = np.ascontiguousarray(sch, dtype=np.float64)
+        return self.L.or_mp2_rhf(n, nocc, ncore, dptr(e), dptr(Cm), g, dptr(sch), cut)
 
 
 def legacy_rhf(oracle, g, n, mol, nE, drag=0.25, scf_max=80):
@@ -346,3 +373,28 @@ class Reference:
         z = np.zeros(1)
         self.L.GTO_JK_Matrix_Quartet_Parallel(0, 0, dptr(z), dptr(z), None, dptr(z), 0.0,
                                               dptr(z), dptr(z), dptr(z), C.byref(opt))
+
+
+class ReferenceMP2:
+    """The reference's MP2 (src/mp2.c with the seven hot-path files, compiled
+    into oracle/_ref/libsqref_mp2.so).  Loaded with lazy binding: the RPC
+    layer that mp2_rhf_aqij_Parallel reaches for nCPU > 1 is left unresolved
+    and is never called with nCPU = 1."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libsqref_mp2.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+        L = self.L = C.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        L.mp2_rhf_aqij_Parallel.argtypes = [I, I, PD, PD, PG, PM, C.POINTER(Option)]
+        L.mp2_rhf_aqij_Parallel.restype = D
+        L.getFrozenCore_RHF.argtypes = [PM]
+        L.getFrozenCore_RHF.restype = I
+
+    def mp2(self, g, n, mol, nocc, e, Cm, max_mem=4000):
+        """mp2_rhf_aqij_Parallel with nCPU = 1 and a memory limit (MB) large
+        enough for one cycle."""
+        opt = default_option(restricted=1, nCPU=1, maxMem=max_mem)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        return self.L.mp2_rhf_aqij_Parallel(n, nocc, dptr(e), dptr(Cm), g, mol, C.byref(opt))

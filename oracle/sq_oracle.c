@@ -20,6 +20,7 @@
  *   J/K digestion       src/matrix.c:2934-3396 (six PQIJ cases)
  *   J/K driver          src/matrix.c:3703-3986
  *   SCF                 src/uhf.c:48-315, 1171-1783; src/conv.c:606-844
+ *   MP2                 src/mp2.c:29-67, 581-712; src/matrix.c:4797-5073
  * How each part is pinned (tests/test_oracle.py unless named):
  *   Boys                mpmath quadrature; recursion invariant; bit-identical
  *                       to the compiled reference (oracle/_ref)
@@ -33,6 +34,10 @@
  *                       monopole substitution)
  *   readers             tests/test_host_c.py
  *   SCF                 GAMESS and the reference's own log energies
+ *   MP2                 the compiled reference's mp2_rhf_aqij_Parallel at its
+ *                       own cut-off; the textbook full-tensor sum over MO
+ *                       integrals; the published H2O/STO-3G MP2 energy
+ *                       (tests/test_oracle_mp2.py)
  * Parity unpinned: eigenvector phases and the DIIS path of the SCF (only
  * converged energies fix them).
  */
@@ -1719,4 +1724,151 @@ void or_free_basis(int n, struct GTOBasis_t *g)
 {
 	for (int i = 0; i < n; i++) { free(g[i].exp); free(g[i].coef); free(g[i].norm); }
 	free(g);
+}
+
+/* ------------------------------------------------------------------ */
+/* MP2 (src/mp2.c:29-67, 581-712; src/matrix.c:4797-5073)             */
+/* ------------------------------------------------------------------ */
+
+/* frozen-core orbitals: half the noble-gas core electrons per atom
+ * (src/mp2.c:29-67) */
+int or_mp2_frozen_core(const struct Molecule_t *m)
+{
+	int core = 0;
+	for (int a = 0; a < m->nAtom; a++) {
+		const int z = m->Z[a];
+		if (z > 54) return -1;
+		core += z <= 2 ? 0 : z <= 10 ? 2 : z <= 18 ? 10 : z <= 36 ? 18 : 36;
+	}
+	return core / 2;
+}
+
+/* the dense pair matrix V[pair(p,q) m + pair(r,s)] = (pq|rs), m = n(n+1)/2,
+ * pair(p,q) = p(p+1)/2 + q for p >= q: every contracted integral, no
+ * screening (the layout the device integral store writes) */
+void or_eri_pairs(int n, const struct GTOBasis_t *g, double *V)
+{
+	const long m = (long)n * (n + 1) / 2;
+	for (int p = 0, P = 0; p < n; p++)
+		for (int q = 0; q <= p; q++, P++)
+			for (int r = 0, Q = 0; r < n && Q <= P; r++)
+				for (int s = 0; s <= r && Q <= P; s++, Q++) {
+					const double v = or_contr_eri(g, p, q, r, s);
+					V[P * m + Q] = v;
+					V[Q * m + P] = v;
+				}
+}
+
+/* One ordered tuple (P,Q,I,J) of the integral EE into the (aq|ij) bank of
+ * mp2_gen_aqij_Quartet (its aqij macro, src/matrix.c:4797-4802): pairs
+ * I >= J that the bank keeps receive C[P, a] EE in row a, column Q. */
+static void or_aqij_add(double *bank, const long *slot, int n, int nCorr, int nCore, const double *C, int P,
+                        int Q, int I, int J, double EE)
+{
+	if (J > I || slot[I * n + J] < 0) return;
+	double *b = bank + slot[I * n + J] * (long)nCorr * n;
+	for (int a = 0; a < nCorr; a++) b[(long)a * n + Q] += C[(long)(nCore + a) * n + P] * EE;
+}
+
+/* Closed-shell MP2 energy by the reference's aqij algorithm for all
+ * correlated orbitals in one cycle (no multipole substitution, which the
+ * reference uses only from its second cycle on):
+ *   (aq|ij) for the kept pairs i >= j (basis Schwarz >= cut), from the
+ *   canonical shell quartets with shell Schwarz product >= cut and the
+ *   integrals |EE| > cut, each unique integral added for the distinct
+ *   orderings of its shell-quartet type (src/matrix.c:4864-5073);
+ *   then per a >= b: (aq|bj) = sum_i C_bi (aq|ij) over |Sch_ij C_bi| > cut,
+ *   (ar|bj), (ar|bs), (as|br) and the r >= s sum of
+ *   2 [(ar|bs)^2 + (as|br)^2 - (ar|bs)(as|br)] / (e_a + e_b - e_r - e_s),
+ *   halved for r = s, doubled for a != b (src/mp2.c:581-712).
+ * cut = 0 drops nothing: the exact MP2 energy.  sch: basis Schwarz (n x n). */
+double or_mp2_rhf(int n, int nOcc, int nCore, const double *e, const double *C, const struct GTOBasis_t *g,
+                  const double *sch, double cut)
+{
+	const int nCorr = nOcc - nCore, nVir = n - nOcc;
+	if (nCorr <= 0 || nVir <= 0) return 0.0;
+	long *slot = malloc(sizeof(long) * n * n), nEE = 0;
+	for (int i = 0; i < n; i++)
+		for (int j = 0; j <= i; j++) {
+			slot[i * n + j] = sch[i * n + j] < cut ? -1 : nEE++;
+			slot[j * n + i] = slot[i * n + j];
+		}
+	double *bank = calloc((size_t)nEE * nCorr * n, sizeof(double));
+	int ns = 0;
+	or_shell *sh = or_make_shells(n, g, &ns);
+	double *ssh = calloc((size_t)ns * ns, sizeof(double));
+	for (int P = 0; P < ns; P++)
+		for (int Q = 0; Q < ns; Q++) {
+			double mx = 0.0;
+			for (int p = sh[P].min; p <= sh[P].max; p++)
+				for (int q = sh[Q].min; q <= sh[Q].max; q++)
+					if (mx < sch[p * n + q]) mx = sch[p * n + q];
+			ssh[P * ns + Q] = mx;
+		}
+	double *EE = malloc(sizeof(double) * OR_MAXB * OR_MAXB * OR_MAXB * OR_MAXB);
+	for (int P = 0; P < ns; P++)
+		for (int Q = 0; Q <= P; Q++)
+			for (int I = 0; I <= P; I++)
+				for (int J = 0; J <= (I == P ? Q : I); J++) {
+					if (ssh[P * ns + Q] * ssh[I * ns + J] < cut) continue;
+					or_quartet(sh + P, sh + Q, sh + I, sh + J, EE);
+					/* orderings of the type: bit k of the mask takes the k-th of
+					 * (pq|ij) (qp|ij) (pq|ji) (qp|ji) (ij|pq) (ji|pq) (ij|qp) (ji|qp) */
+					int mask;
+					if (P == Q && I == J && P == I) mask = 0x01;        /* one shell */
+					else if (P == Q && I == J) mask = 0x11;             /* two diagonal pairs */
+					else if (P == Q) mask = 0x35;                       /* bra diagonal */
+					else if (I == J) mask = 0x53;                       /* ket diagonal */
+					else if (P == I && Q == J) mask = 0x0F;             /* bra pair = ket pair */
+					else mask = 0xFF;
+					const double *v = EE;
+					for (int p = sh[P].min; p <= sh[P].max; p++)
+						for (int q = sh[Q].min; q <= sh[Q].max; q++)
+							for (int i = sh[I].min; i <= sh[I].max; i++)
+								for (int j = sh[J].min; j <= sh[J].max; j++, v++) {
+									if (!(fabs(*v) > cut)) continue;
+									const int t[8][4] = {{p, q, i, j}, {q, p, i, j}, {p, q, j, i}, {q, p, j, i},
+									                     {i, j, p, q}, {j, i, p, q}, {i, j, q, p}, {j, i, q, p}};
+									for (int k = 0; k < 8; k++)
+										if (mask >> k & 1)
+											or_aqij_add(bank, slot, n, nCorr, nCore, C, t[k][0], t[k][1], t[k][2],
+											            t[k][3], *v);
+								}
+				}
+	free(EE);
+	double *aqbj = malloc(sizeof(double) * n * n), *arbj = malloc(sizeof(double) * nVir * n);
+	const double *eCor = e + nCore, *eVir = e + nOcc;
+	double sum = 0.0;
+	for (int a = 0; a < nCorr; a++)
+		for (int b = 0; b <= a; b++) {
+			memset(aqbj, 0, sizeof(double) * n * n);
+			for (int i = 0; i < n; i++)
+				for (int j = 0; j < n; j++) {
+					if (slot[i * n + j] < 0) continue;
+					const double cbi = C[(long)(b + nCore) * n + i];
+					if (!(fabs(sch[i * n + j] * cbi) > cut)) continue;
+					const double *src = bank + slot[i * n + j] * (long)nCorr * n + (long)a * n;
+					for (int q = 0; q < n; q++) aqbj[j * n + q] += cbi * src[q];
+				}
+			for (int r = 0; r < nVir; r++)
+				for (int j = 0; j < n; j++) {
+					double s = 0.0;
+					for (int q = 0; q < n; q++) s += C[(long)(r + nOcc) * n + q] * aqbj[j * n + q];
+					arbj[r * n + j] = s;
+				}
+			for (int r = 0; r < nVir; r++)
+				for (int s = 0; s <= r; s++) {
+					double arbs = 0.0, asbr = 0.0;
+					for (int j = 0; j < n; j++) {
+						arbs += C[(long)(s + nOcc) * n + j] * arbj[r * n + j];
+						asbr += C[(long)(r + nOcc) * n + j] * arbj[s * n + j];
+					}
+					double c = (arbs * arbs + asbr * asbr - arbs * asbr) * 2.0 / (eCor[a] + eCor[b] - eVir[r] - eVir[s]);
+					if (r == s) c *= 0.5;
+					if (a != b) c *= 2.0;
+					sum += c;
+				}
+		}
+	free(aqbj); free(arbj); free(bank); free(slot); free(sh); free(ssh);
+	return sum;
 }

@@ -65,7 +65,9 @@ struct Tile {
   int seg, r0, r1, k0, v0, v1;
 };
 
-enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2 };
+// kStore writes the contracted integrals of each quartet into a dense
+// pair-by-pair matrix (the MP2 transform's input) instead of digesting them.
+enum Mode { kRHF = 0, kUHF = 1, kSchwarz = 2, kStore = 3 };
 
 // Per-molecule data of a batched launch (one entry per molecule): its
 // matrices, screening maxima, half-accumulators and, per ket pair class,
@@ -232,6 +234,22 @@ __host__ __device__ __forceinline__ void fatomic(double* p, double v) {
 #else
   *p += v;  // host instantiation (used only to check the kernel arithmetic on the CPU)
 #endif
+}
+
+// Packed index of the basis-function pair (p, q), p >= q after ordering:
+// p (p + 1) / 2 + q.
+__host__ __device__ __forceinline__ size_t pair_index(int p, int q) {
+  return p >= q ? (size_t)p * (p + 1) / 2 + q : (size_t)q * (q + 1) / 2 + p;
+}
+
+// Integral store of kStore: a plain store when the quartet's whole primitive
+// sum is in hand, an atomic add of a partial sum otherwise (a segment whose
+// ket primitive pairs are split over several tiles).
+__host__ __device__ __forceinline__ void eri_put(double* base, size_t off, double v, bool whole) {
+  if (whole)
+    base[off] = v;
+  else
+    fatomic(base + off, v);
 }
 
 // Lane permutation of a double by a DPP control (both 32-bit halves).
@@ -409,6 +427,33 @@ __host__ __device__ __forceinline__ void quartet(const ClassArgs& A, const View&
       const double v = sqrt(fabs((double)acc[ab][cs] * sc * sc));
       A.sch_out[(fa + ia) * n + fb + jb] = v;
       A.sch_out[(fb + jb) * n + fa + ia] = v;
+    });
+    return;
+  } else if constexpr (MODE == kStore) {
+    // (pq|rs) into V[pair(p,q)][pair(r,s)] and V[pair(r,s)][pair(p,q)] of the
+    // n(n+1)/2 x n(n+1)/2 matrix at sch_out; a pair of one sub-shell with
+    // itself stores p >= q only, a quartet of one pair with itself one of the
+    // two positions (its component loop visits both)
+    if (!valid) return;
+    const size_t ld = (size_t)n * (n + 1) / 2;
+    const bool sb = bp.a == bp.b, sk = kp.a == kp.b, diag = A.same && ib == ik;
+    const bool whole = v0 == 0 && v1 == kp.npp;
+    sfor<NS>([&](auto csI) {
+      constexpr int cs = decltype(csI)::value;
+      constexpr int cd = CD0 + cs;
+      const int r = fc + cd / ND, s = fd + cd % ND;
+      if (sk && s > r) return;
+      const double scd = A.bscale[gc + cd / ND] * A.bscale[gd + cd % ND];
+      const size_t Q = pair_index(r, s);
+      sfor<NAB>([&](auto abI) {
+        constexpr int ab = decltype(abI)::value;
+        const int p = fa + ab / NB, q = fb + ab % NB;
+        if (sb && q > p) return;
+        const size_t P = pair_index(p, q);
+        const double v = (double)acc[ab][cs] * scd * A.bscale[ga + ab / NB] * A.bscale[gb + ab % NB];
+        eri_put(A.sch_out, P * ld + Q, v, whole);
+        if (!diag) eri_put(A.sch_out, Q * ld + P, v, whole);
+      });
     });
     return;
   } else {

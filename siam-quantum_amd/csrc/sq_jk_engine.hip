@@ -37,8 +37,10 @@
 
 #include "../../include/sq_jk.h"
 #include "../../include/sq_grad.h"
+#include "../../include/sq_mp2.h"
 #include "sq_jk_classes.h"
 #include "sq_grad_classes.h"
+#include "sq_mp2_classes.h"
 #include "sq_jk_wd.h"
 #include "sq_one_e.h"
 #include "sq_internal.h"
@@ -108,6 +110,14 @@ GradLauncher grad_launcher_for(int la, int lb, int lc, int ld) {
   if (la == a && lb == b && lc == c && ld == d) return &sqgrad_launch_##a##b##c##d;
   SQJK_CLASSES(SQGRAD_CASE)
 #undef SQGRAD_CASE
+  return nullptr;
+}
+
+StoreLauncher store_launcher_for(int la, int lb, int lc, int ld) {
+#define SQMP2_CASE(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return &sqmp2_store_##a##b##c##d;
+  SQJK_CLASSES(SQMP2_CASE)
+#undef SQMP2_CASE
   return nullptr;
 }
 
@@ -1681,6 +1691,122 @@ int sqjk_grad_eri(sqjk_ctx* c, const double* PA, const double* PB, double Kfacto
     const int at = basis2Atom[c->sub[k].bf_off];
     for (int x = 0; x < 3; ++x) G[3 * at + x] += gs[3 * k + x];
   }
+  return SQJK_OK;
+}
+
+// Integrals of every shell quartet of the plan at `cutoff` (no density
+// screening) into the dense pair matrix dV (m x m, m = n(n+1)/2), zeroed
+// here.  Returns the quartets evaluated and the kernel time.
+static int store_pairs(sqjk_ctx* c, double cutoff, double* dV, long long* quartets, float* ms) {
+  const size_t m = (size_t)c->n * (c->n + 1) / 2;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for((long long)std::min<size_t>(m * m, (size_t)1 << 40), 256)), dim3(256), 0,
+                     c->st, dV, m * m, 0.0);
+  HIPCHK(hipGetLastError());
+  int rc;
+  if ((rc = make_plan(c, cutoff, cutoff))) return rc;
+  HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (const sqjk_ctx::Launch& L : c->launches) {
+    ClassArgs a = base_args(c);
+    a.bra = c->d_pairs[L.X];
+    a.ket = c->d_pairs[L.Y];
+    a.seg = c->d_seg;
+    a.tile = c->d_tile + L.tile0;
+    a.rbra = c->d_rbra;
+    a.rlo = c->d_rlo;
+    a.rhi = c->d_rhi;
+    a.same = (L.X == L.Y);
+    a.total = L.ntile;
+    a.fixed_cut = 0.0;
+    a.npb = (int)c->pairs[L.X].size();
+    a.npk = (int)c->pairs[L.Y].size();
+    a.sch_out = dV;
+    StoreLauncher f = store_launcher_for(kPairL[L.X][0], kPairL[L.X][1], kPairL[L.Y][0], kPairL[L.Y][1]);
+    if (!f || f(a, c->st) < 0) return fail(SQJK_EHIP, "integral store kernel launch failed");
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  *quartets = c->plan_q;
+  return SQJK_OK;
+}
+
+int sqjk_eri_pairs(sqjk_ctx* c, double cutoff, double* V) {
+  if (!c || !V || !(cutoff >= 0.0)) return fail(SQJK_EARG, "sqjk_eri_pairs: invalid arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t m = (size_t)c->n * (c->n + 1) / 2;
+  double* dV = nullptr;
+  if (hipMalloc(&dV, m * m * 8) != hipSuccess)
+    return fail(SQJK_ENOMEM, "sqjk_eri_pairs: the pair matrix needs " + std::to_string(m * m * 8 >> 20) + " MiB");
+  long long q = 0;
+  float ms = 0.0f;
+  int rc = store_pairs(c, cutoff, dV, &q, &ms);
+  if (rc == SQJK_OK) {
+    hipError_t e = hipMemcpy(V, dV, m * m * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(SQJK_EHIP, std::string("sqjk_eri_pairs: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(dV);
+  c->last.kernel_ms = ms;
+  return rc;
+}
+
+int sqjk_mp2_rhf(sqjk_ctx* c, int nOcc, int nCore, const double* e, const double* C, double cutoff,
+                 sqjk_mp2_info* info) {
+  if (!c || !e || !C || !(cutoff >= 0.0)) return fail(SQJK_EARG, "sqjk_mp2_rhf: invalid arguments");
+  const int n = c->n;
+  if (nCore < 0 || nOcc < nCore || nOcc > n) return fail(SQJK_EARG, "sqjk_mp2_rhf: need 0 <= nCore <= nOcc <= nBasis");
+  HIPCHK(hipSetDevice(c->device));
+  const int nc = nOcc - nCore, nv = n - nOcc;
+  sqjk_mp2_info r{};
+  r.device_bytes = (long long)sq_mp2_bytes(n, nc, nv);
+  if (nc > 0 && nv > 0) {
+    size_t freeb = 0, totb = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &totb));
+    const size_t need = (size_t)r.device_bytes + ((size_t)1 << 30);
+    if (need > freeb)
+      return fail(SQJK_ENOMEM, "sqjk_mp2_rhf: needs " + std::to_string(need >> 20) + " MiB of device memory (the pair " +
+                                   "matrix and the half-transformed integrals), " + std::to_string(freeb >> 20) +
+                                   " MiB free");
+    const size_t m = (size_t)n * (n + 1) / 2;
+    double *dV = nullptr, *dCo = nullptr, *dCv = nullptr, *dec = nullptr, *dev = nullptr;
+    auto release = [&] {
+      for (double* p : {dV, dCo, dCv, dec, dev})
+        if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&dV, m * m * 8) != hipSuccess || hipMalloc(&dCo, (size_t)n * nc * 8) != hipSuccess ||
+        hipMalloc(&dCv, (size_t)n * nv * 8) != hipSuccess || hipMalloc(&dec, (size_t)nc * 8) != hipSuccess ||
+        hipMalloc(&dev, (size_t)nv * 8) != hipSuccess) {
+      release();
+      return fail(SQJK_ENOMEM, "sqjk_mp2_rhf: device allocation failed");
+    }
+    // orbital-major C: the correlated and virtual orbitals are contiguous
+    // column blocks of the column-major n x n matrix
+    hipError_t he = hipMemcpyAsync(dCo, C + (size_t)nCore * n, (size_t)n * nc * 8, hipMemcpyHostToDevice, c->st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dCv, C + (size_t)nOcc * n, (size_t)n * nv * 8, hipMemcpyHostToDevice, c->st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dec, e + nCore, (size_t)nc * 8, hipMemcpyHostToDevice, c->st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dev, e + nOcc, (size_t)nv * 8, hipMemcpyHostToDevice, c->st);
+    if (he != hipSuccess) {
+      release();
+      return fail(SQJK_EHIP, std::string("sqjk_mp2_rhf: upload failed: ") + hipGetErrorString(he));
+    }
+    float ms = 0.0f;
+    int rc = store_pairs(c, cutoff, dV, &r.quartets, &ms);
+    if (rc) {
+      release();
+      return rc;
+    }
+    r.store_ms = ms;
+    SqMp2Result t{};
+    std::string err;
+    rc = sq_mp2_from_pairs(c->st, n, nc, nv, dCo, dCv, dec, dev, &dV, &t, &err);
+    release();
+    if (rc) return fail(SQJK_EHIP, "sqjk_mp2_rhf: " + err);
+    r.e2 = t.e2;
+    r.half1_ms = t.half1_ms;
+    r.half2_ms = t.half2_ms;
+    r.half1_flop = t.half1_flop;
+    r.half2_flop = t.half2_flop;
+  }
+  if (info) *info = r;
   return SQJK_OK;
 }
 

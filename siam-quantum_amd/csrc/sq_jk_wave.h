@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
       if (j < uload(A.rlo + r) || j >= uload(A.rhi + r)) continue;  // uniform
       ib = uload(A.rbra + r);
       ik = ket0 + j;
-      if (!density_screen_pass(V, uload(A.bra + ib), uload(A.ket + ik))) continue;  // uniform
+      if constexpr (MODE != kStore)
+        if (!density_screen_pass(V, uload(A.bra + ib), uload(A.ket + ik))) continue;  // uniform
     }
     const PairRec bp = uload(A.bra + ib), kp = uload(A.ket + ik);
     const SubShell sA = uload(A.sh + bp.a), sB = uload(A.sh + bp.b), sC = uload(A.sh + kp.a), sD = uload(A.sh + kp.b);
@@ -273,6 +274,22 @@ __global__ void __launch_bounds__(kWT, 2) k_class_wave(ClassArgs A) {
           }
         }
       }
+    } else if constexpr (MODE == kStore) {
+      // the per-thread kernel's kStore rule, one entry per lane and step
+      const size_t ld = (size_t)n * (n + 1) / 2;
+      const bool sb = bp.a == bp.b, sk = kp.a == kp.b, diag = A.same && ib == ik;
+      const bool whole = v0 == 0 && v1 == kp.npp;
+      for (int o = lane; o < NINT; o += kWT) {
+        const int ab = o / NCD, cd = o - ab * NCD;
+        const int p = fa + ab / NB, q = fb + ab % NB, rr = fc + cd / ND, ss = fd + cd % ND;
+        if ((sb && q > p) || (sk && ss > rr)) continue;
+        const double v = (double)acc[o] * A.bscale[ga + ab / NB] * A.bscale[gb + ab % NB] * A.bscale[gc + cd / ND] *
+                         A.bscale[gd + cd % ND];
+        const size_t P = pair_index(p, q), Q = pair_index(rr, ss);
+        eri_put(A.sch_out, P * ld + Q, v, whole);
+        if (!diag) eri_put(A.sch_out, Q * ld + P, v, whole);
+      }
+      wave_sync();
     } else {
       double deg = 1.0;
       if (bp.a == bp.b) deg *= 0.5;

@@ -21,6 +21,13 @@ class SqjkStats(C.Structure):
                 ("quartets_fp32", C.c_longlong), ("launches", C.c_int)]
 
 
+class Mp2Info(C.Structure):
+    """sqjk_mp2_info (include/sq_mp2.h)."""
+    _fields_ = [("e2", C.c_double), ("store_ms", C.c_double), ("half1_ms", C.c_double), ("half2_ms", C.c_double),
+                ("half1_flop", C.c_double), ("half2_flop", C.c_double), ("quartets", C.c_longlong),
+                ("device_bytes", C.c_longlong)]
+
+
 def lib():
     """Load lib/libsqamd.so once; raise if it has not been built."""
     global _lib
@@ -142,6 +149,28 @@ class JKEngine:
                                b2a.ctypes.data_as(P32), int(natom), dptr(G)), "sqjk_grad_eri")
         return G
 
+    def eri_pairs(self, cutoff=0.0):
+        """Dense pair matrix V[pair(p,q), pair(r,s)] = (pq|rs) of every shell
+        quartet above the Schwarz cut-off (sqjk_eri_pairs)."""
+        L = lib()
+        L.sqjk_eri_pairs.argtypes = [C.c_void_p, C.c_double, PD]
+        m = self.n * (self.n + 1) // 2
+        V = np.zeros((m, m))
+        _check(L.sqjk_eri_pairs(self._h, float(cutoff), dptr(V)), "sqjk_eri_pairs")
+        return V
+
+    def mp2_rhf(self, nocc, ncore, e, Cm, cutoff=0.0):
+        """Closed-shell MP2 correlation energy and timings (sqjk_mp2_rhf);
+        Cm: orbital-major coefficients C[k*n + i]."""
+        L = lib()
+        L.sqjk_mp2_rhf.argtypes = [C.c_void_p, C.c_int, C.c_int, PD, PD, C.c_double, C.POINTER(Mp2Info)]
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        info = Mp2Info()
+        _check(L.sqjk_mp2_rhf(self._h, int(nocc), int(ncore), dptr(e), dptr(Cm), float(cutoff), C.byref(info)),
+               "sqjk_mp2_rhf")
+        return {f: getattr(info, f) for f, _ in Mp2Info._fields_}
+
     def class_times(self):
         """Kernel ms per class of the last build (profile mode), class_stats order."""
         t = (C.c_double * 21)()
@@ -202,3 +231,23 @@ def uhf_force(bas, mol, nEA, nEB, CA, CB, eA, eB, opt):
     L.uhf_force(bas.n, bas.ptr, mol.ptr, nEA, nEB, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt),
                 dptr(F[0]), dptr(F[1]), dptr(F[2]))
     return F.T.copy()
+
+
+def mp2_rhf_aqij_parallel(bas, mol, nocc, e, Cm, opt):
+    """The mp2_rhf_aqij_Parallel drop-in: frozen-core RHF MP2 energy
+    correction (prints the reference's report)."""
+    from .abi import Molecule
+    L = lib()
+    L.mp2_rhf_aqij_Parallel.argtypes = [C.c_int, C.c_int, PD, PD, PG, C.POINTER(Molecule), C.POINTER(Option)]
+    L.mp2_rhf_aqij_Parallel.restype = C.c_double
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+    return L.mp2_rhf_aqij_Parallel(bas.n, int(nocc), dptr(e), dptr(Cm), bas.ptr, mol.ptr, C.byref(opt))
+
+
+def frozen_core(mol):
+    """getFrozenCore_RHF of the drop-in library."""
+    from .abi import Molecule
+    L = lib()
+    L.getFrozenCore_RHF.argtypes = [C.POINTER(Molecule)]
+    return L.getFrozenCore_RHF(mol.ptr)

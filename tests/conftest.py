@@ -31,6 +31,16 @@ def reference_grad():
 
 
 @pytest.fixture(scope="session")
+def reference_mp2():
+    """The reference's MP2 (src/mp2.c) compiled into oracle/_ref (if built)."""
+    from oracle import ReferenceMP2
+    try:
+        return ReferenceMP2()
+    except FileNotFoundError:
+        pytest.skip("oracle/_ref/libsqref_mp2.so not built")
+
+
+@pytest.fixture(scope="session")
 def reference():
     """The reference's own hot-path code compiled into oracle/_ref (if built)."""
     from oracle import Reference

@@ -1,0 +1,119 @@
+"""MP2 on the device (sqjk_eri_pairs, sqjk_mp2_rhf and the
+mp2_rhf_aqij_Parallel drop-in) against the oracle (the reference's aqij
+algorithm restated in sq_oracle.c, itself pinned against the compiled
+reference, the textbook MO-integral sum and the published H2O/STO-3G MP2
+energy on the CPU).
+
+Tolerances: integrals of the device store equal the oracle's contracted
+integrals to 1e-12 (FP64, different recursions); the unscreened device MP2
+energy equals the oracle's exact energy to 1e-10 Eh; at the Schwarz cut-off
+1e-10 of the drop-in the device stays within 1e-9 Eh of it, and within
+1e-9 Eh of the reference's 1e-8 rules.
+"""
+import numpy as np
+import pytest
+
+from molecules import g2
+from sqamd import JKEngine, default_option, inputs
+from sqamd.abi import Mol
+from sqamd.engine import frozen_core, mp2_rhf_aqij_parallel
+
+pytestmark = pytest.mark.gpu
+CRAWFORD_BOHR = np.array([[0.0, -0.143225816552, 0.0], [1.638036840407, 1.136548822547, 0.0],
+                          [-1.638036840407, 1.136548822547, 0.0]])
+
+
+def _rhf(oracle, mol, bas, conv=1e-10):
+    ne = mol.n_electrons
+    _, _, (CA, _, eA, _) = oracle.uhf(bas.ptr, bas.n, mol.ptr, ne // 2, ne // 2, scf_conv=conv, want_orbitals=True)
+    return CA, eA
+
+
+@pytest.mark.parametrize("name,basis", [("h2o_q0m1", "631g_star"), ("nh3_q0m1", "631g_star"), ("h2o_q0m1", "sto3g")])
+def test_eri_pairs_vs_oracle(oracle, name, basis):
+    """Every contracted integral of the store kernels (all classes of s, p,
+    d shells and the L-shell split of STO-3G) against the oracle's."""
+    mol, bas = g2(name, basis)
+    V = JKEngine(bas.ptr, bas.n).eri_pairs(0.0)
+    R = oracle.eri_pairs(bas.ptr, bas.n)
+    err = np.abs(V - R).max()
+    print(f"{name}/{basis}: m={V.shape[0]} max|dV| = {err:.2e} (max|V| {np.abs(R).max():.3f})")
+    assert err < 1e-12
+    # a quartet of one pair with itself evaluates (pq|rs) and (rs|pq) as two
+    # components, equal up to rounding
+    assert np.abs(V - V.T).max() < 1e-13
+
+
+def test_eri_pairs_dd_classes(oracle):
+    """Ethylene 6-31G*: (dd|dd) on the wavefront kernel, primitive-split tiles."""
+    mol, bas = g2("c2h4_q0m1", "631g_star")
+    V = JKEngine(bas.ptr, bas.n).eri_pairs(0.0)
+    R = oracle.eri_pairs(bas.ptr, bas.n)
+    err = np.abs(V - R).max()
+    print(f"c2h4/6-31G*: max|dV| = {err:.2e}")
+    assert err < 1e-12
+
+
+def test_mp2_water_sto3g_published(oracle):
+    """All-electron H2O/STO-3G at the published geometry: -0.049149636120 Eh."""
+    mol = Mol([8, 1, 1], CRAWFORD_BOHR)
+    bas = inputs.make_basis(mol, "sto3g")
+    Cm, e = _rhf(oracle, mol, bas, 1e-12)
+    r = JKEngine(bas.ptr, bas.n).mp2_rhf(5, 0, e, Cm, 0.0)
+    print(f"E_MP2 = {r['e2']:.12f}")
+    assert abs(r["e2"] - (-0.049149636120)) < 1e-11
+
+
+@pytest.mark.parametrize("name", ["h2o_q0m1", "c2h4_q0m1", "ch3cl_q0m1"])
+def test_mp2_exact_vs_oracle(oracle, name):
+    mol, bas = g2(name, "631g_star")
+    ne = mol.n_electrons
+    Cm, e = _rhf(oracle, mol, bas)
+    nc = oracle.mp2_frozen_core(mol.ptr)
+    r = JKEngine(bas.ptr, bas.n).mp2_rhf(ne // 2, nc, e, Cm, 0.0)
+    x = oracle.mp2_rhf(bas.ptr, bas.n, ne // 2, nc, e, Cm, 0.0)
+    print(f"{name}: n={bas.n} device {r['e2']:.14f} oracle {x:.14f} d={r['e2'] - x:.1e}")
+    assert abs(r["e2"] - x) < 1e-10
+
+
+def test_mp2_water_dimer_screened(oracle):
+    """Two molecules (38 functions): the screened device energy (1e-10)
+    against the exact and the reference-rules oracle energies."""
+    Z, X = inputs.water_cluster(2, 7)
+    mol = inputs.mol_from_angstrom(Z, X)
+    bas = inputs.make_basis(mol, "631g_star")
+    Cm, e = _rhf(oracle, mol, bas)
+    jk = JKEngine(bas.ptr, bas.n)
+    ex = jk.mp2_rhf(10, 2, e, Cm, 0.0)["e2"]
+    sc = jk.mp2_rhf(10, 2, e, Cm, 1e-10)["e2"]
+    x = oracle.mp2_rhf(bas.ptr, bas.n, 10, 2, e, Cm, 0.0)
+    rules = oracle.mp2_rhf(bas.ptr, bas.n, 10, 2, e, Cm, 1e-8)
+    print(f"exact {ex:.14f} screened {sc:.14f} oracle {x:.14f} reference rules {rules:.14f}")
+    assert abs(ex - x) < 1e-10
+    assert abs(sc - x) < 1e-9
+    assert abs(sc - rules) < 1e-9
+
+
+def test_mp2_no_correlated_orbitals():
+    mol, bas = g2("h2o_q0m1", "sto3g")
+    n = bas.n
+    r = JKEngine(bas.ptr, n).mp2_rhf(5, 5, np.zeros(n), np.eye(n), 0.0)
+    assert r["e2"] == 0.0
+
+
+def test_mp2_dropin_vs_oracle(oracle, capfd):
+    """mp2_rhf_aqij_Parallel: frozen core, the reference's report lines, the
+    energy of the reference's rules within 1e-9 Eh."""
+    mol, bas = g2("c2h4_q0m1", "631g_star")
+    ne = mol.n_electrons
+    Cm, e = _rhf(oracle, mol, bas)
+    assert frozen_core(mol) == oracle.mp2_frozen_core(mol.ptr) == 2
+    e2 = mp2_rhf_aqij_parallel(bas, mol, ne // 2, e, Cm, default_option(restricted=1))
+    out = capfd.readouterr().out
+    rules = oracle.mp2_rhf(bas.ptr, bas.n, ne // 2, 2, e, Cm, 1e-8)
+    print(out)
+    print(f"drop-in {e2:.14f} reference rules {rules:.14f}")
+    assert "MP2 ENERGY CORRECTION" in out
+    assert "The number of excluded core orbitals:          %4d" % 2 in out
+    assert f"MP2 Energy Correction is {rules:12.6f} Hartree" in out
+    assert abs(e2 - rules) < 1e-9
