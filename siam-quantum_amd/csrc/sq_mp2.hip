@@ -11,9 +11,13 @@
 // DGEMMs, so X[(a,r)][j] = (ar|ls).  Second half (per block of virtual
 // orbitals s): K[j][(b,s)] = Co[l,b] Cv[s',s] + Co[s',b] Cv[l,s] for
 // j = pair(l,s') (one term when l = s'), and Y = X K, Y[(a,r)][(b,s)] =
-// (ar|bs), one DGEMM on the matrix cores; the energy kernel then sums
-//   E2 = sum_{a,b,r,s} (ar|bs) [2 (ar|bs) - (as|br)] / (e_a + e_b - e_r - e_s),
-// reading (as|br) = Y[(b,r)][(a,s)] from the same column block.
+// (ar|bs), one DGEMM on the matrix cores.  The energy
+//   E2 = sum_{a,b,r,s} (ar|bs) [2 (ar|bs) - (as|br)] / (e_a + e_b - e_r - e_s)
+// has the same (r,s) and (s,r) terms (swap a<->b, r<->s), so a block of
+// virtual orbitals s in [s0, s1) needs only the rows r >= s0 of Y: it sums
+// the r > s terms twice and the r = s terms once, reading (as|br) =
+// Y[(b,r)][(a,s)] from the same block.  With eight or more blocks this
+// computes a little over half of Y.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -54,20 +58,24 @@ __global__ void k_kmat(const double* __restrict__ Co, const double* __restrict__
   }
 }
 
-// part[block] += sum over the column block of Y of (ar|bs)[2(ar|bs) - (as|br)] / D
+// part[block] += sum over the block of Y (rows r >= s0, columns s in
+// [s0, s0 + ns), leading dimension ld = (nv - s0) nc) of
+// w_rs (ar|bs)[2(ar|bs) - (as|br)] / D, w = 2 for r > s, 1 for r = s, 0 below
 constexpr int kEnergyBlock = 256;
 __global__ void __launch_bounds__(kEnergyBlock) k_energy(const double* __restrict__ Y, int nc, int nv, int s0, int ns,
                                                         const double* __restrict__ ec, const double* __restrict__ ev,
                                                         double* __restrict__ part) {
-  const size_t ncv = (size_t)nc * nv, total = ncv * (size_t)ns * nc;
+  const size_t ld = (size_t)(nv - s0) * nc, total = ld * (size_t)ns * nc;
   double acc = 0.0;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t col = e / ncv, row = e - col * ncv;
-    const int r = (int)(row / nc), a = (int)(row % nc);
+    const size_t col = e / ld, row = e - col * ld;
+    const int rl = (int)(row / nc), a = (int)(row % nc);
     const int sl = (int)(col / nc), b = (int)(col % nc);
+    if (rl < sl) continue;
     const double v = Y[e];
-    const double x = Y[((size_t)sl * nc + a) * ncv + (size_t)r * nc + b];
-    acc += v * (2.0 * v - x) / (ec[a] + ec[b] - ev[r] - ev[s0 + sl]);
+    const double x = Y[((size_t)sl * nc + a) * ld + (size_t)rl * nc + b];
+    const double w = rl == sl ? 1.0 : 2.0;
+    acc += w * v * (2.0 * v - x) / (ec[a] + ec[b] - ev[s0 + rl] - ev[s0 + sl]);
   }
   __shared__ double red[kEnergyBlock];
   red[threadIdx.x] = acc;
@@ -189,22 +197,26 @@ int sq_mp2_from_pairs(hipStream_t st, int n, int nc, int nv, const double* dCo, 
   (void)hipFree(*dV);
   *dV = nullptr;
 
-  // second half, blocks of virtual orbitals s: K (npr x ns nc), Y (ncv x ns nc)
+  // second half, blocks of virtual orbitals s: K (npr x ns nc), Y (rows r >=
+  // s0: at most ncv x ns nc); at least eight blocks, more if memory asks
   if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return bad("hipMemGetInfo failed");
   const size_t per_s = (size_t)nc * (npr + ncv) * 8;
-  const int ns = (int)std::max<size_t>(1, std::min<size_t>((size_t)nv, (freeb / 2) / per_s));
+  const int ns = (int)std::max<size_t>(1, std::min<size_t>({(size_t)(nv + 7) / 8, (size_t)nv, (freeb / 2) / per_s}));
   double *dK = nullptr, *dY = nullptr;
   Bufs blk;
   if (!blk.get(&dK, npr * (size_t)ns * nc) || !blk.get(&dY, ncv * (size_t)ns * nc))
     return bad("device allocation of the second-half blocks failed");
+  double flop2 = 0.0;
   for (int s0 = 0; s0 < nv; s0 += ns) {
     const int sc = std::min(ns, nv - s0);
-    const size_t cols = (size_t)sc * nc;
+    const size_t cols = (size_t)sc * nc, rows = (size_t)(nv - s0) * nc;
     hipLaunchKernelGGL(k_kmat, dim3(grid_of(npr * cols, 256)), dim3(256), 0, st, dCo, dCv, dLM, n, nc, s0, sc, npr, dK);
-    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)ncv, (rocblas_int)cols,
-                      (rocblas_int)npr, &one, dX, (rocblas_int)ncv, dK, (rocblas_int)npr, &zero, dY,
-                      (rocblas_int)ncv) != rocblas_status_success)
+    // Y = X[rows (a, r >= s0)] K   (rows x cols)
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)rows, (rocblas_int)cols,
+                      (rocblas_int)npr, &one, dX + (size_t)s0 * nc, (rocblas_int)ncv, dK, (rocblas_int)npr, &zero, dY,
+                      (rocblas_int)rows) != rocblas_status_success)
       return bad("rocblas_dgemm (second half) failed");
+    flop2 += 2.0 * (double)rows * cols * npr;
     hipLaunchKernelGGL(k_energy, dim3(G), dim3(kEnergyBlock), 0, st, dY, nc, nv, s0, sc, dec, dev, dPart);
   }
   if (hipEventRecord(e2, st) != hipSuccess) return bad("hipEventRecord failed");
@@ -221,6 +233,6 @@ int sq_mp2_from_pairs(hipStream_t st, int n, int nc, int nv, const double* dCo, 
   out->half1_ms = ms1;
   out->half2_ms = ms2;
   out->half1_flop = 2.0 * nc * (double)n * n * npr + 2.0 * nc * (double)nv * n * npr;
-  out->half2_flop = 2.0 * (double)ncv * ncv * npr;
+  out->half2_flop = flop2;
   return 0;
 }
