@@ -2,7 +2,7 @@
  * sq_mp2.h - closed-shell MP2 correlation energy on the device.
  *
  * Drop-ins for src/mp2.h (mp2_rhf_aqij_Parallel, mp2_rhf_aqij,
- * getFrozenCore_RHF) and the context-level entry points under them.  The
+ * mp2_uhf_aqij, getFrozenCore_RHF) and the context-level entry points under them.  The
  * integrals come from the J/K context's pair lists and work plan (every
  * shell quartet above the Schwarz cut-off, no density screening), written
  * into a dense matrix over basis-function pairs; the four-index transform is
@@ -49,6 +49,16 @@ int sqjk_eri_pairs(sqjk_ctx *ctx, double cutoff, double *V);
 int sqjk_mp2_rhf(sqjk_ctx *ctx, int nOcc, int nCore, const double *e, const double *C, double cutoff,
                  sqjk_mp2_info *info);
 
+/* Unrestricted MP2 correlation energy (src/mp2.c:2126-2432):
+ *   E2 = sum_{a<b, r<s} [(ar|bs) - (as|br)]^2 / D  over alpha orbitals
+ *      + the same over beta orbitals
+ *      + sum_{a, r alpha; b, s beta} (ar|bs)^2 / D,   D = e_a + e_b - e_r - e_s,
+ * with the first nCore orbitals of each spin frozen.  CA, CB orbital-major,
+ * eA, eB their energies.  The pair matrix is evaluated once for both spins;
+ * device memory (n(n+1)/2)^2 + n(n+1)/2 (ncA nvA + ncB nvB) doubles. */
+int sqjk_mp2_uhf(sqjk_ctx *ctx, int nEA, int nEB, int nCore, const double *eA, const double *eB, const double *CA,
+                 const double *CB, double cutoff, sqjk_mp2_info *info);
+
 /* ---- drop-in surface (reference signatures, src/mp2.h) ---- */
 
 /* Frozen-core orbitals: half the noble-gas core electrons of every atom
@@ -68,6 +78,13 @@ double mp2_rhf_aqij_Parallel(int nBasis, int nOcc, const double *e, const double
 /* src/mp2.c:344: the serial variant, the same result. */
 double mp2_rhf_aqij(int nBasis, int nOcc, const double *e, const double *C, const struct GTOBasis_t *gto,
                     const struct Molecule_t *mol, const struct option_t *opt);
+
+/* src/mp2.c:2126-2432: frozen-core unrestricted MP2 energy correction,
+ * printed in the reference's report format and returned; both spins in one
+ * device pass.  Shell quartets below the Schwarz product 1e-10 are left out. */
+double mp2_uhf_aqij(int nBasis, int nEA, int nEB, const double *eA, const double *eB, const double *CA,
+                    const double *CB, const struct GTOBasis_t *gto, const struct Molecule_t *mol,
+                    const struct option_t *opt);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,8 @@ class Oracle:
         L.or_eri_pairs.argtypes = [I, PG, PD]
         L.or_mp2_rhf.argtypes = [I, I, I, PD, PD, PG, PD, D]
         L.or_mp2_rhf.restype = D
+        L.or_mp2_uhf.argtypes = [I, I, I, I, PD, PD, PD, PD, PG, PD, D]
+        L.or_mp2_uhf.restype = D
 
     # -- scalar pieces -------------------------------------------------
     def boys(self, mmax, T):
@@ -227,6 +229,14 @@ class Oracle:
         Cm = np.ascontiguousarray(Cm, dtype=np.float64)
         sch = np.ascontiguousarray(sch, dtype=np.float64)
         return self.L.or_mp2_rhf(n, nocc, ncore, dptr(e), dptr(Cm), g, dptr(sch), cut)
+
+    def mp2_uhf(self, g, n, nea, neb, ncore, eA, eB, CA, CB, cut=0.0, sch=None):
+        """Unrestricted MP2 energy by the reference's algorithm (both spins'
+        banks in one cycle); cut = 0 is exact."""
+        if sch is None:
+            sch = self.schwarz(g, n)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eA, eB, CA, CB, sch)]
+        return self.L.or_mp2_uhf(n, nea, neb, ncore, *[dptr(x) for x in a[:4]], g, dptr(a[4]), cut)
 
 
 def legacy_rhf(oracle, g, n, mol, nE, drag=0.25, scf_max=80):
@@ -390,6 +400,8 @@ class ReferenceMP2:
         L.mp2_rhf_aqij_Parallel.restype = D
         L.getFrozenCore_RHF.argtypes = [PM]
         L.getFrozenCore_RHF.restype = I
+        L.mp2_uhf_aqij.argtypes = [I, I, I, PD, PD, PD, PD, PG, PM, C.POINTER(Option)]
+        L.mp2_uhf_aqij.restype = D
 
     def mp2(self, g, n, mol, nocc, e, Cm, max_mem=4000):
         """mp2_rhf_aqij_Parallel with nCPU = 1 and a memory limit (MB) large
@@ -398,3 +410,9 @@ class ReferenceMP2:
         e = np.ascontiguousarray(e, dtype=np.float64)
         Cm = np.ascontiguousarray(Cm, dtype=np.float64)
         return self.L.mp2_rhf_aqij_Parallel(n, nocc, dptr(e), dptr(Cm), g, mol, C.byref(opt))
+
+    def mp2_uhf(self, g, n, mol, nea, neb, eA, eB, CA, CB, max_mem=4000):
+        """mp2_uhf_aqij with a memory limit (MB) large enough for one cycle per spin."""
+        opt = default_option(restricted=0, nCPU=1, maxMem=max_mem)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eA, eB, CA, CB)]
+        return self.L.mp2_uhf_aqij(n, nea, neb, *[dptr(x) for x in a], g, mol, C.byref(opt))

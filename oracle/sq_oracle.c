@@ -1770,30 +1770,16 @@ static void or_aqij_add(double *bank, const long *slot, int n, int nCorr, int nC
 	for (int a = 0; a < nCorr; a++) b[(long)a * n + Q] += C[(long)(nCore + a) * n + P] * EE;
 }
 
-/* Closed-shell MP2 energy by the reference's aqij algorithm for all
- * correlated orbitals in one cycle (no multipole substitution, which the
- * reference uses only from its second cycle on):
- *   (aq|ij) for the kept pairs i >= j (basis Schwarz >= cut), from the
- *   canonical shell quartets with shell Schwarz product >= cut and the
- *   integrals |EE| > cut, each unique integral added for the distinct
- *   orderings of its shell-quartet type (src/matrix.c:4864-5073);
- *   then per a >= b: (aq|bj) = sum_i C_bi (aq|ij) over |Sch_ij C_bi| > cut,
- *   (ar|bj), (ar|bs), (as|br) and the r >= s sum of
- *   2 [(ar|bs)^2 + (as|br)^2 - (ar|bs)(as|br)] / (e_a + e_b - e_r - e_s),
- *   halved for r = s, doubled for a != b (src/mp2.c:581-712).
- * cut = 0 drops nothing: the exact MP2 energy.  sch: basis Schwarz (n x n). */
-double or_mp2_rhf(int n, int nOcc, int nCore, const double *e, const double *C, const struct GTOBasis_t *g,
-                  const double *sch, double cut)
+/* The (aq|ij) bank of mp2_gen_aqij_Quartet for the correlated orbitals
+ * [nCore, nCore + nCorr) of C (src/matrix.c:4864-5073, one cycle, no
+ * multipole substitution): rows slot[i*n+j] * nCorr + a of n doubles, for
+ * the kept pairs i >= j (basis Schwarz >= cut); canonical shell quartets with
+ * shell Schwarz product >= cut, integrals |EE| > cut, each unique integral
+ * added for the distinct orderings of its shell-quartet type. */
+static double *or_mp2_bank(int n, int nCorr, int nCore, const double *C, const struct GTOBasis_t *g,
+                           const double *sch, double cut, const long *slot, long nEE)
 {
-	const int nCorr = nOcc - nCore, nVir = n - nOcc;
-	if (nCorr <= 0 || nVir <= 0) return 0.0;
-	long *slot = malloc(sizeof(long) * n * n), nEE = 0;
-	for (int i = 0; i < n; i++)
-		for (int j = 0; j <= i; j++) {
-			slot[i * n + j] = sch[i * n + j] < cut ? -1 : nEE++;
-			slot[j * n + i] = slot[i * n + j];
-		}
-	double *bank = calloc((size_t)nEE * nCorr * n, sizeof(double));
+	double *bank = calloc((size_t)(nEE > 0 ? nEE : 1) * nCorr * n, sizeof(double));
 	int ns = 0;
 	or_shell *sh = or_make_shells(n, g, &ns);
 	double *ssh = calloc((size_t)ns * ns, sizeof(double));
@@ -1835,33 +1821,72 @@ double or_mp2_rhf(int n, int nOcc, int nCore, const double *e, const double *C, 
 											            t[k][3], *v);
 								}
 				}
-	free(EE);
+	free(EE); free(sh); free(ssh);
+	return bank;
+}
+
+/* (ar|bj) for one bank row a and occupied orbital cb (coefficients of b):
+ * (aq|bj) = sum_i C_bi (aq|ij) over |Sch_ij C_bi| > cut, then
+ * arbj[r*n + j] = sum_q Cv[r][q] (aq|bj) for the nVir virtual orbitals Cv
+ * (src/mp2.c:653-671, 2240-2262) */
+static void or_mp2_arbj(int n, const double *bank, int nCorr, int a, const long *slot, const double *cb,
+                        const double *sch, double cut, const double *Cv, int nVir, double *aqbj, double *arbj)
+{
+	memset(aqbj, 0, sizeof(double) * n * n);
+	for (int i = 0; i < n; i++)
+		for (int j = 0; j < n; j++) {
+			if (slot[i * n + j] < 0) continue;
+			const double cbi = cb[i];
+			if (!(fabs(sch[i * n + j] * cbi) > cut)) continue;
+			const double *src = bank + (slot[i * n + j] * nCorr + a) * (long)n;
+			for (int q = 0; q < n; q++) aqbj[j * n + q] += cbi * src[q];
+		}
+	for (int r = 0; r < nVir; r++)
+		for (int j = 0; j < n; j++) {
+			double s = 0.0;
+			for (int q = 0; q < n; q++) s += Cv[(long)r * n + q] * aqbj[j * n + q];
+			arbj[r * n + j] = s;
+		}
+}
+
+static long *or_mp2_slots(int n, const double *sch, double cut, long *nEE)
+{
+	long *slot = malloc(sizeof(long) * n * n), k = 0;
+	for (int i = 0; i < n; i++)
+		for (int j = 0; j <= i; j++) {
+			slot[i * n + j] = sch[i * n + j] < cut ? -1 : k++;
+			slot[j * n + i] = slot[i * n + j];
+		}
+	*nEE = k;
+	return slot;
+}
+
+/* Closed-shell MP2 energy by the reference's aqij algorithm for all
+ * correlated orbitals in one cycle (src/mp2.c:581-712): per a >= b the
+ * (ar|bs), (as|br) and the r >= s sum of
+ *   2 [(ar|bs)^2 + (as|br)^2 - (ar|bs)(as|br)] / (e_a + e_b - e_r - e_s),
+ * halved for r = s, doubled for a != b.  cut = 0 drops nothing: the exact
+ * MP2 energy.  sch: basis Schwarz (n x n). */
+double or_mp2_rhf(int n, int nOcc, int nCore, const double *e, const double *C, const struct GTOBasis_t *g,
+                  const double *sch, double cut)
+{
+	const int nCorr = nOcc - nCore, nVir = n - nOcc;
+	if (nCorr <= 0 || nVir <= 0) return 0.0;
+	long nEE;
+	long *slot = or_mp2_slots(n, sch, cut, &nEE);
+	double *bank = or_mp2_bank(n, nCorr, nCore, C, g, sch, cut, slot, nEE);
 	double *aqbj = malloc(sizeof(double) * n * n), *arbj = malloc(sizeof(double) * nVir * n);
-	const double *eCor = e + nCore, *eVir = e + nOcc;
+	const double *eCor = e + nCore, *eVir = e + nOcc, *Cv = C + (long)nOcc * n;
 	double sum = 0.0;
 	for (int a = 0; a < nCorr; a++)
 		for (int b = 0; b <= a; b++) {
-			memset(aqbj, 0, sizeof(double) * n * n);
-			for (int i = 0; i < n; i++)
-				for (int j = 0; j < n; j++) {
-					if (slot[i * n + j] < 0) continue;
-					const double cbi = C[(long)(b + nCore) * n + i];
-					if (!(fabs(sch[i * n + j] * cbi) > cut)) continue;
-					const double *src = bank + slot[i * n + j] * (long)nCorr * n + (long)a * n;
-					for (int q = 0; q < n; q++) aqbj[j * n + q] += cbi * src[q];
-				}
-			for (int r = 0; r < nVir; r++)
-				for (int j = 0; j < n; j++) {
-					double s = 0.0;
-					for (int q = 0; q < n; q++) s += C[(long)(r + nOcc) * n + q] * aqbj[j * n + q];
-					arbj[r * n + j] = s;
-				}
+			or_mp2_arbj(n, bank, nCorr, a, slot, C + (long)(b + nCore) * n, sch, cut, Cv, nVir, aqbj, arbj);
 			for (int r = 0; r < nVir; r++)
 				for (int s = 0; s <= r; s++) {
 					double arbs = 0.0, asbr = 0.0;
 					for (int j = 0; j < n; j++) {
-						arbs += C[(long)(s + nOcc) * n + j] * arbj[r * n + j];
-						asbr += C[(long)(r + nOcc) * n + j] * arbj[s * n + j];
+						arbs += Cv[(long)s * n + j] * arbj[r * n + j];
+						asbr += Cv[(long)r * n + j] * arbj[s * n + j];
 					}
 					double c = (arbs * arbs + asbr * asbr - arbs * asbr) * 2.0 / (eCor[a] + eCor[b] - eVir[r] - eVir[s]);
 					if (r == s) c *= 0.5;
@@ -1869,6 +1894,60 @@ double or_mp2_rhf(int n, int nOcc, int nCore, const double *e, const double *C, 
 					sum += c;
 				}
 		}
-	free(aqbj); free(arbj); free(bank); free(slot); free(sh); free(ssh);
+	free(aqbj); free(arbj); free(bank); free(slot);
+	return sum;
+}
+
+/* Unrestricted MP2 energy by the reference's algorithm (src/mp2.c:2126-2432,
+ * each spin's bank in one cycle, no multipole substitution): for each spin
+ * the a < b, r < s sum of [(ar|bs) - (as|br)]^2 / D over its own orbitals,
+ * and the alpha-beta sum of (ar|bs)^2 / D with a, r alpha and b, s beta. */
+double or_mp2_uhf(int n, int nEA, int nEB, int nCore, const double *eA, const double *eB, const double *CA,
+                  const double *CB, const struct GTOBasis_t *g, const double *sch, double cut)
+{
+	const int nOcc[2] = {nEA, nEB};
+	const double *e[2] = {eA, eB}, *C[2] = {CA, CB};
+	long nEE;
+	long *slot = or_mp2_slots(n, sch, cut, &nEE);
+	double *aqbj = malloc(sizeof(double) * n * n), *arbj = malloc(sizeof(double) * n * n);
+	double sum = 0.0;
+	for (int sp = 0; sp < 2; sp++) {
+		const int nCorr = nOcc[sp] - nCore, nVir = n - nOcc[sp];
+		if (nCorr <= 0 || nVir <= 0) continue;
+		double *bank = or_mp2_bank(n, nCorr, nCore, C[sp], g, sch, cut, slot, nEE);
+		const double *eCor = e[sp] + nCore, *eVir = e[sp] + nOcc[sp], *Cv = C[sp] + (long)nOcc[sp] * n;
+		for (int a = 0; a < nCorr; a++) {
+			/* same spin, b > a, r < s */
+			for (int b = a + 1; b < nCorr; b++) {
+				or_mp2_arbj(n, bank, nCorr, a, slot, C[sp] + (long)(b + nCore) * n, sch, cut, Cv, nVir, aqbj, arbj);
+				for (int r = 0; r < nVir; r++)
+					for (int s = r + 1; s < nVir; s++) {
+						double arbs = 0.0, asbr = 0.0;
+						for (int j = 0; j < n; j++) {
+							arbs += Cv[(long)s * n + j] * arbj[r * n + j];
+							asbr += Cv[(long)r * n + j] * arbj[s * n + j];
+						}
+						const double d = arbs - asbr;
+						sum += d * d / (eCor[a] + eCor[b] - eVir[r] - eVir[s]);
+					}
+			}
+			/* opposite spin, from the alpha bank only */
+			if (sp == 0) {
+				const int nCorrB = nEB - nCore, nVirB = n - nEB;
+				const double *eCorB = eB + nCore, *eVirB = eB + nEB, *CvB = CB + (long)nEB * n;
+				for (int b = 0; b < nCorrB; b++) {
+					or_mp2_arbj(n, bank, nCorr, a, slot, CB + (long)(b + nCore) * n, sch, cut, Cv, nVir, aqbj, arbj);
+					for (int r = 0; r < nVir; r++)
+						for (int s = 0; s < nVirB; s++) {
+							double arbs = 0.0;
+							for (int j = 0; j < n; j++) arbs += CvB[(long)s * n + j] * arbj[r * n + j];
+							sum += arbs * arbs / (eCor[a] + eCorB[b] - eVir[r] - eVirB[s]);
+						}
+				}
+			}
+		}
+		free(bank);
+	}
+	free(aqbj); free(arbj); free(slot);
 	return sum;
 }

@@ -1749,44 +1749,64 @@ int sqjk_eri_pairs(sqjk_ctx* c, double cutoff, double* V) {
   return rc;
 }
 
-int sqjk_mp2_rhf(sqjk_ctx* c, int nOcc, int nCore, const double* e, const double* C, double cutoff,
-                 sqjk_mp2_info* info) {
-  if (!c || !e || !C || !(cutoff >= 0.0)) return fail(SQJK_EARG, "sqjk_mp2_rhf: invalid arguments");
+// MP2 of one (closed shell) or two (alpha, beta) spin blocks: nOcc[k]
+// occupied orbitals of C[k] (orbital-major), the first nCore frozen.
+static int mp2_run(sqjk_ctx* c, int nspin, const int* nOcc, int nCore, const double* const* e, const double* const* C,
+                   double cutoff, sqjk_mp2_info* info, const char* who) {
   const int n = c->n;
-  if (nCore < 0 || nOcc < nCore || nOcc > n) return fail(SQJK_EARG, "sqjk_mp2_rhf: need 0 <= nCore <= nOcc <= nBasis");
+  for (int k = 0; k < nspin; ++k) {
+    if (!e[k] || !C[k]) return fail(SQJK_EARG, std::string(who) + ": invalid arguments");
+    if (nCore < 0 || nOcc[k] < nCore || nOcc[k] > n)
+      return fail(SQJK_EARG, std::string(who) + ": need 0 <= nCore <= nOcc <= nBasis");
+  }
   HIPCHK(hipSetDevice(c->device));
-  const int nc = nOcc - nCore, nv = n - nOcc;
   sqjk_mp2_info r{};
-  r.device_bytes = (long long)sq_mp2_bytes(n, nc, nv);
-  if (nc > 0 && nv > 0) {
+  int nc[2] = {0, 0}, nv[2] = {0, 0};
+  size_t ncv = 0;
+  bool any = false;
+  for (int k = 0; k < nspin; ++k) {
+    nc[k] = nOcc[k] - nCore;
+    nv[k] = n - nOcc[k];
+    ncv += (size_t)nc[k] * nv[k];
+    any = any || (nc[k] > 0 && nv[k] > 0);
+  }
+  r.device_bytes = (long long)sq_mp2_bytes(n, ncv);
+  if (any) {
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
     const size_t need = (size_t)r.device_bytes + ((size_t)1 << 30);
     if (need > freeb)
-      return fail(SQJK_ENOMEM, "sqjk_mp2_rhf: needs " + std::to_string(need >> 20) + " MiB of device memory (the pair " +
-                                   "matrix and the half-transformed integrals), " + std::to_string(freeb >> 20) +
-                                   " MiB free");
+      return fail(SQJK_ENOMEM, std::string(who) + ": needs " + std::to_string(need >> 20) +
+                                   " MiB of device memory (the pair matrix and the half-transformed integrals), " +
+                                   std::to_string(freeb >> 20) + " MiB free");
     const size_t m = (size_t)n * (n + 1) / 2;
-    double *dV = nullptr, *dCo = nullptr, *dCv = nullptr, *dec = nullptr, *dev = nullptr;
+    std::vector<double*> bufs;
+    double* dV = nullptr;
     auto release = [&] {
-      for (double* p : {dV, dCo, dCv, dec, dev})
-        if (p) (void)hipFree(p);
+      if (dV) (void)hipFree(dV);
+      for (double* p : bufs) (void)hipFree(p);
     };
-    if (hipMalloc(&dV, m * m * 8) != hipSuccess || hipMalloc(&dCo, (size_t)n * nc * 8) != hipSuccess ||
-        hipMalloc(&dCv, (size_t)n * nv * 8) != hipSuccess || hipMalloc(&dec, (size_t)nc * 8) != hipSuccess ||
-        hipMalloc(&dev, (size_t)nv * 8) != hipSuccess) {
-      release();
-      return fail(SQJK_ENOMEM, "sqjk_mp2_rhf: device allocation failed");
+    auto up = [&](const double* h, size_t count) -> double* {
+      double* d = nullptr;
+      if (hipMalloc(&d, std::max<size_t>(count, 1) * 8) != hipSuccess) return nullptr;
+      bufs.push_back(d);
+      if (count && hipMemcpyAsync(d, h, count * 8, hipMemcpyHostToDevice, c->st) != hipSuccess) return nullptr;
+      return d;
+    };
+    SqMp2Spin sp[2] = {};
+    bool ok = hipMalloc(&dV, m * m * 8) == hipSuccess;
+    for (int k = 0; ok && k < nspin; ++k) {
+      // orbital-major C: the correlated and virtual orbitals are contiguous
+      // column blocks of the column-major n x n matrix
+      sp[k].nc = nc[k];
+      sp[k].nv = nv[k];
+      ok = (sp[k].dCo = up(C[k] + (size_t)nCore * n, (size_t)n * nc[k])) &&
+           (sp[k].dCv = up(C[k] + (size_t)nOcc[k] * n, (size_t)n * nv[k])) &&
+           (sp[k].dec = up(e[k] + nCore, (size_t)nc[k])) && (sp[k].dev = up(e[k] + nOcc[k], (size_t)nv[k]));
     }
-    // orbital-major C: the correlated and virtual orbitals are contiguous
-    // column blocks of the column-major n x n matrix
-    hipError_t he = hipMemcpyAsync(dCo, C + (size_t)nCore * n, (size_t)n * nc * 8, hipMemcpyHostToDevice, c->st);
-    if (he == hipSuccess) he = hipMemcpyAsync(dCv, C + (size_t)nOcc * n, (size_t)n * nv * 8, hipMemcpyHostToDevice, c->st);
-    if (he == hipSuccess) he = hipMemcpyAsync(dec, e + nCore, (size_t)nc * 8, hipMemcpyHostToDevice, c->st);
-    if (he == hipSuccess) he = hipMemcpyAsync(dev, e + nOcc, (size_t)nv * 8, hipMemcpyHostToDevice, c->st);
-    if (he != hipSuccess) {
+    if (!ok) {
       release();
-      return fail(SQJK_EHIP, std::string("sqjk_mp2_rhf: upload failed: ") + hipGetErrorString(he));
+      return fail(SQJK_ENOMEM, std::string(who) + ": device allocation or upload failed");
     }
     float ms = 0.0f;
     int rc = store_pairs(c, cutoff, dV, &r.quartets, &ms);
@@ -1797,9 +1817,9 @@ int sqjk_mp2_rhf(sqjk_ctx* c, int nOcc, int nCore, const double* e, const double
     r.store_ms = ms;
     SqMp2Result t{};
     std::string err;
-    rc = sq_mp2_from_pairs(c->st, n, nc, nv, dCo, dCv, dec, dev, &dV, &t, &err);
+    rc = sq_mp2_from_pairs(c->st, n, nspin, sp, &dV, &t, &err);
     release();
-    if (rc) return fail(SQJK_EHIP, "sqjk_mp2_rhf: " + err);
+    if (rc) return fail(SQJK_EHIP, std::string(who) + ": " + err);
     r.e2 = t.e2;
     r.half1_ms = t.half1_ms;
     r.half2_ms = t.half2_ms;
@@ -1808,6 +1828,23 @@ int sqjk_mp2_rhf(sqjk_ctx* c, int nOcc, int nCore, const double* e, const double
   }
   if (info) *info = r;
   return SQJK_OK;
+}
+
+int sqjk_mp2_rhf(sqjk_ctx* c, int nOcc, int nCore, const double* e, const double* C, double cutoff,
+                 sqjk_mp2_info* info) {
+  if (!c || !(cutoff >= 0.0)) return fail(SQJK_EARG, "sqjk_mp2_rhf: invalid arguments");
+  const double* ee[1] = {e};
+  const double* cc[1] = {C};
+  return mp2_run(c, 1, &nOcc, nCore, ee, cc, cutoff, info, "sqjk_mp2_rhf");
+}
+
+int sqjk_mp2_uhf(sqjk_ctx* c, int nEA, int nEB, int nCore, const double* eA, const double* eB, const double* CA,
+                 const double* CB, double cutoff, sqjk_mp2_info* info) {
+  if (!c || !(cutoff >= 0.0)) return fail(SQJK_EARG, "sqjk_mp2_uhf: invalid arguments");
+  const int no[2] = {nEA, nEB};
+  const double* ee[2] = {eA, eB};
+  const double* cc[2] = {CA, CB};
+  return mp2_run(c, 2, no, nCore, ee, cc, cutoff, info, "sqjk_mp2_uhf");
 }
 
 int sqjk_grad_one_electron(int nBasis, const GTOBasis_t* gto, const Molecule_t* mol, const double* PT,

@@ -58,24 +58,39 @@ __global__ void k_kmat(const double* __restrict__ Co, const double* __restrict__
   }
 }
 
-// part[block] += sum over the block of Y (rows r >= s0, columns s in
-// [s0, s0 + ns), leading dimension ld = (nv - s0) nc) of
-// w_rs (ar|bs)[2(ar|bs) - (as|br)] / D, w = 2 for r > s, 1 for r = s, 0 below
+// part[block] += sum over one block of Y = (ar|bs) (columns: ket virtual
+// orbitals s in [s0, s0 + ns) x ket occupied b; rows: bra virtual r x bra
+// occupied a, from r = r0) of
+//   MODE 0 (closed shell, rows r >= s0):  w (ar|bs)[2(ar|bs) - (as|br)] / D,
+//          w = 2 for r > s, 1 for r = s (the (s,r) terms equal the (r,s) ones)
+//   MODE 1 (one spin, rows r >= s0):      [(ar|bs) - (as|br)]^2 / (2 D), r > s
+//   MODE 2 (two spins, all rows):         (ar|bs)^2 / D
+// with D = e_a + e_b - e_r - e_s.
 constexpr int kEnergyBlock = 256;
-__global__ void __launch_bounds__(kEnergyBlock) k_energy(const double* __restrict__ Y, int nc, int nv, int s0, int ns,
-                                                        const double* __restrict__ ec, const double* __restrict__ ev,
-                                                        double* __restrict__ part) {
-  const size_t ld = (size_t)(nv - s0) * nc, total = ld * (size_t)ns * nc;
+template <int MODE>
+__global__ void __launch_bounds__(kEnergyBlock) k_energy(const double* __restrict__ Y, int nca, int nva, int ncb,
+                                                        int s0, int ns, const double* __restrict__ eca,
+                                                        const double* __restrict__ eva, const double* __restrict__ ecb,
+                                                        const double* __restrict__ evb, double* __restrict__ part) {
+  const int r0 = MODE == 2 ? 0 : s0;
+  const size_t ld = (size_t)(nva - r0) * nca, total = ld * (size_t)ns * ncb;
   double acc = 0.0;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
     const size_t col = e / ld, row = e - col * ld;
-    const int rl = (int)(row / nc), a = (int)(row % nc);
-    const int sl = (int)(col / nc), b = (int)(col % nc);
-    if (rl < sl) continue;
+    const int rl = (int)(row / nca), a = (int)(row % nca);
+    const int sl = (int)(col / ncb), b = (int)(col % ncb);
     const double v = Y[e];
-    const double x = Y[((size_t)sl * nc + a) * ld + (size_t)rl * nc + b];
-    const double w = rl == sl ? 1.0 : 2.0;
-    acc += w * v * (2.0 * v - x) / (ec[a] + ec[b] - ev[s0 + rl] - ev[s0 + sl]);
+    const double d = eca[a] + ecb[b] - eva[r0 + rl] - evb[s0 + sl];
+    if constexpr (MODE == 2) {
+      acc += v * v / d;
+    } else {
+      if (rl < sl || (MODE == 1 && rl == sl)) continue;
+      const double x = Y[((size_t)sl * nca + a) * ld + (size_t)rl * nca + b];
+      if constexpr (MODE == 0)
+        acc += (rl == sl ? 1.0 : 2.0) * v * (2.0 * v - x) / d;
+      else
+        acc += 0.5 * (v - x) * (v - x) / d;
+    }
   }
   __shared__ double red[kEnergyBlock];
   red[threadIdx.x] = acc;
@@ -111,119 +126,175 @@ struct Bufs {
 
 }  // namespace
 
-size_t sq_mp2_bytes(int n, int nc, int nv) {
-  const size_t npr = (size_t)n * (n + 1) / 2, ncv = (size_t)nc * nv;
-  return npr * npr * 8 + npr * ncv * 8;
+size_t sq_mp2_bytes(int n, size_t ncv_sum) {
+  const size_t npr = (size_t)n * (n + 1) / 2;
+  return npr * npr * 8 + npr * ncv_sum * 8;
 }
 
-int sq_mp2_from_pairs(hipStream_t st, int n, int nc, int nv, const double* dCo, const double* dCv, const double* dec,
-                      const double* dev, double** dV, SqMp2Result* out, std::string* err) {
-  auto bad = [&](const std::string& m) {
+namespace {
+
+constexpr int kParts = 1024;  // energy partial sums (one per workgroup)
+
+struct Ctx {
+  hipStream_t st;
+  rocblas_handle h;
+  int n;
+  size_t npr;
+  const int2* lm;
+  double* part;
+  std::string* err;
+  int bad(const std::string& m) {
+    *err = m;
+    return -1;
+  }
+};
+
+// X[(a,r)][j] = (ar|ls) for every ket pair j = (l,s) of the pair matrix V:
+// per block of pairs, unpack and two DGEMMs (Co^T M, then T Cv)
+int first_half(Ctx& c, const double* dV, const SqMp2Spin& sp, double* dX) {
+  const int n = c.n, nc = sp.nc, nv = sp.nv;
+  const size_t npr = c.npr, ncv = (size_t)nc * nv;
+  const double one = 1.0, zero = 0.0;
+  size_t freeb = 0, totb = 0;
+  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return c.bad("hipMemGetInfo failed");
+  const size_t per_pair = (size_t)n * n * 8 + (size_t)nc * n * 8;
+  const size_t blk_budget = std::min<size_t>(freeb / 4, (size_t)2 << 30);
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>({npr, blk_budget / per_pair, (size_t)INT32_MAX / n}));
+  double *dM = nullptr, *dT = nullptr;
+  Bufs blk;
+  if (!blk.get(&dM, (size_t)nb * n * n) || !blk.get(&dT, (size_t)nc * nb * n))
+    return c.bad("device allocation of the unpack blocks failed");
+  for (size_t j0 = 0; j0 < npr; j0 += nb) {
+    const int bc = (int)std::min<size_t>(nb, npr - j0);
+    hipLaunchKernelGGL(k_unpack, dim3(grid_of((size_t)bc * n * n, 256)), dim3(256), 0, c.st, dV, npr, (long long)j0, bc,
+                       n, dM);
+    // T = Co^T [M_0 ... M_bc-1]   (nc x bc n)
+    if (rocblas_dgemm(c.h, rocblas_operation_transpose, rocblas_operation_none, nc, bc * n, n, &one, sp.dCo, n, dM, n,
+                      &zero, dT, nc) != rocblas_status_success)
+      return c.bad("rocblas_dgemm (first quarter) failed");
+    // X_j = T_j Cv   (nc x nv each)
+    if (rocblas_dgemm_strided_batched(c.h, rocblas_operation_none, rocblas_operation_none, nc, nv, n, &one, dT, nc,
+                                      (rocblas_stride)nc * n, sp.dCv, n, 0, &zero, dX + j0 * ncv, nc,
+                                      (rocblas_stride)ncv, bc) != rocblas_status_success)
+      return c.bad("rocblas_dgemm_strided_batched (second quarter) failed");
+  }
+  if (hipStreamSynchronize(c.st) != hipSuccess)
+    return c.bad(std::string("first half-transform failed: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+// Y = X_bra K_ket block by block over the ket virtual orbitals, summed into
+// c.part by k_energy<MODE> (MODE 0/1: bra and ket the same spin, rows r >= s0
+// only; MODE 2: all rows)
+template <int MODE>
+int second_half(Ctx& c, const double* dX, const SqMp2Spin& bra, const SqMp2Spin& ket, double* flop) {
+  const size_t npr = c.npr, ncv = (size_t)bra.nc * bra.nv;
+  const double one = 1.0, zero = 0.0;
+  size_t freeb = 0, totb = 0;
+  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return c.bad("hipMemGetInfo failed");
+  const size_t per_s = (size_t)ket.nc * (npr + ncv) * 8;
+  const size_t nblk_min = MODE == 2 ? 1 : 8;  // blocks restrict the rows only for one spin
+  const int ns = (int)std::max<size_t>(
+      1, std::min<size_t>({((size_t)ket.nv + nblk_min - 1) / nblk_min, (size_t)ket.nv, (freeb / 2) / per_s}));
+  double *dK = nullptr, *dY = nullptr;
+  Bufs blk;
+  if (!blk.get(&dK, npr * (size_t)ns * ket.nc) || !blk.get(&dY, ncv * (size_t)ns * ket.nc))
+    return c.bad("device allocation of the second-half blocks failed");
+  for (int s0 = 0; s0 < ket.nv; s0 += ns) {
+    const int sc = std::min(ns, ket.nv - s0);
+    const int r0 = MODE == 2 ? 0 : s0;
+    const size_t cols = (size_t)sc * ket.nc, rows = (size_t)(bra.nv - r0) * bra.nc;
+    hipLaunchKernelGGL(k_kmat, dim3(grid_of(npr * cols, 256)), dim3(256), 0, c.st, ket.dCo, ket.dCv, c.lm, c.n, ket.nc,
+                       s0, sc, npr, dK);
+    // Y = X[rows (a, r >= r0)] K   (rows x cols)
+    if (rocblas_dgemm(c.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)rows, (rocblas_int)cols,
+                      (rocblas_int)npr, &one, dX + (size_t)r0 * bra.nc, (rocblas_int)ncv, dK, (rocblas_int)npr, &zero,
+                      dY, (rocblas_int)rows) != rocblas_status_success)
+      return c.bad("rocblas_dgemm (second half) failed");
+    *flop += 2.0 * (double)rows * cols * npr;
+    hipLaunchKernelGGL(k_energy<MODE>, dim3(kParts), dim3(kEnergyBlock), 0, c.st, dY, bra.nc, bra.nv, ket.nc, s0, sc,
+                       bra.dec, bra.dev, ket.dec, ket.dev, c.part);
+  }
+  if (hipStreamSynchronize(c.st) != hipSuccess)
+    return c.bad(std::string("second half-transform failed: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+}  // namespace
+
+int sq_mp2_from_pairs(hipStream_t st, int n, int nspin, const SqMp2Spin* sp, double** dV, SqMp2Result* out,
+                      std::string* err) {
+  *out = SqMp2Result{};
+  const size_t npr = (size_t)n * (n + 1) / 2;
+  auto fail = [&](const std::string& m) {
     *err = m;
     return -1;
   };
-  *out = SqMp2Result{};
-  const size_t npr = (size_t)n * (n + 1) / 2, ncv = (size_t)nc * nv;
-  if (nc <= 0 || nv <= 0) {  // nothing to correlate
+  bool any = false;
+  for (int k = 0; k < nspin; ++k) any = any || (sp[k].nc > 0 && sp[k].nv > 0);
+  if (!any) {  // nothing to correlate
     (void)hipFree(*dV);
     *dV = nullptr;
     return 0;
   }
   rocblas_handle h = nullptr;
-  if (rocblas_create_handle(&h) != rocblas_status_success) return bad("rocblas_create_handle failed");
+  if (rocblas_create_handle(&h) != rocblas_status_success) return fail("rocblas_create_handle failed");
   struct HandleGuard {
     rocblas_handle h;
     ~HandleGuard() { (void)rocblas_destroy_handle(h); }
   } hg{h};
-  if (rocblas_set_stream(h, st) != rocblas_status_success) return bad("rocblas_set_stream failed");
-
+  if (rocblas_set_stream(h, st) != rocblas_status_success) return fail("rocblas_set_stream failed");
   hipEvent_t e0, e1, e2;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess)
-    return bad("hipEventCreate failed");
+    return fail("hipEventCreate failed");
   struct EventGuard {
     hipEvent_t a, b, c;
     ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
   } eg{e0, e1, e2};
 
   Bufs keep;  // buffers of the whole call
-  double* dX = nullptr;
+  double* dX[2] = {nullptr, nullptr};
   double* dPart = nullptr;
   int2* dLM = nullptr;
-  const int G = 1024;  // energy partial sums (one per workgroup)
-  if (!keep.get(&dX, npr * ncv) || !keep.get(&dPart, G) || !keep.get(&dLM, npr))
-    return bad("device allocation of the half-transformed integrals failed (" +
-               std::to_string(npr * ncv * 8 / 1048576) + " MiB)");
+  if (!keep.get(&dPart, kParts) || !keep.get(&dLM, npr)) return fail("device allocation failed");
+  for (int k = 0; k < nspin; ++k)
+    if (!keep.get(&dX[k], npr * (size_t)sp[k].nc * sp[k].nv))
+      return fail("device allocation of the half-transformed integrals failed (" +
+                  std::to_string(npr * (size_t)sp[k].nc * sp[k].nv * 8 / 1048576) + " MiB)");
   {
     std::vector<int2> lm(npr);
     for (int l = 0, j = 0; l < n; ++l)
       for (int m = 0; m <= l; ++m, ++j) lm[j] = make_int2(l, m);
     if (hipMemcpyAsync(dLM, lm.data(), npr * sizeof(int2), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
-      return bad("pair table upload failed");
+      return fail("pair table upload failed");
   }
-  hipLaunchKernelGGL(k_zero, dim3(4), dim3(256), 0, st, dPart, (size_t)G);
-  const double one = 1.0, zero = 0.0;
+  hipLaunchKernelGGL(k_zero, dim3(4), dim3(256), 0, st, dPart, (size_t)kParts);
+  Ctx c{st, h, n, npr, dLM, dPart, err};
 
-  // first half, blocks of ket pairs: M (nb n^2) and T (nc nb n)
-  size_t freeb = 0, totb = 0;
-  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return bad("hipMemGetInfo failed");
-  const size_t per_pair = (size_t)n * n * 8 + (size_t)nc * n * 8;
-  const size_t blk_budget = std::min<size_t>(freeb / 4, (size_t)2 << 30);
-  const int nb = (int)std::max<size_t>(1, std::min<size_t>({npr, blk_budget / per_pair, (size_t)INT32_MAX / n}));
-  double *dM = nullptr, *dT = nullptr;
-  {
-    Bufs blk;
-    if (!blk.get(&dM, (size_t)nb * n * n) || !blk.get(&dT, (size_t)nc * nb * n))
-      return bad("device allocation of the unpack blocks failed");
-    if (hipEventRecord(e0, st) != hipSuccess) return bad("hipEventRecord failed");
-    for (size_t j0 = 0; j0 < npr; j0 += nb) {
-      const int bc = (int)std::min<size_t>(nb, npr - j0);
-      hipLaunchKernelGGL(k_unpack, dim3(grid_of((size_t)bc * n * n, 256)), dim3(256), 0, st, *dV, npr, (long long)j0,
-                         bc, n, dM);
-      // T = Co^T [M_0 ... M_bc-1]   (nc x bc n)
-      if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, bc * n, n, &one, dCo, n, dM, n,
-                        &zero, dT, nc) != rocblas_status_success)
-        return bad("rocblas_dgemm (first quarter) failed");
-      // X_j = T_j Cv   (nc x nv each)
-      if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nc, nv, n, &one, dT, nc,
-                                        (rocblas_stride)nc * n, dCv, n, 0, &zero, dX + j0 * ncv, nc,
-                                        (rocblas_stride)ncv, bc) != rocblas_status_success)
-        return bad("rocblas_dgemm_strided_batched (second quarter) failed");
-    }
-    if (hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-      return bad(std::string("first half-transform failed: ") + hipGetErrorString(hipGetLastError()));
-  }
-  // V is no longer needed
-  (void)hipFree(*dV);
+  if (hipEventRecord(e0, st) != hipSuccess) return fail("hipEventRecord failed");
+  for (int k = 0; k < nspin; ++k)
+    if (sp[k].nc > 0 && sp[k].nv > 0)
+      if (int rc = first_half(c, *dV, sp[k], dX[k])) return rc;
+  if (hipEventRecord(e1, st) != hipSuccess) return fail("hipEventRecord failed");
+  (void)hipFree(*dV);  // V is no longer needed
   *dV = nullptr;
 
-  // second half, blocks of virtual orbitals s: K (npr x ns nc), Y (rows r >=
-  // s0: at most ncv x ns nc); at least eight blocks, more if memory asks
-  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return bad("hipMemGetInfo failed");
-  const size_t per_s = (size_t)nc * (npr + ncv) * 8;
-  const int ns = (int)std::max<size_t>(1, std::min<size_t>({(size_t)(nv + 7) / 8, (size_t)nv, (freeb / 2) / per_s}));
-  double *dK = nullptr, *dY = nullptr;
-  Bufs blk;
-  if (!blk.get(&dK, npr * (size_t)ns * nc) || !blk.get(&dY, ncv * (size_t)ns * nc))
-    return bad("device allocation of the second-half blocks failed");
   double flop2 = 0.0;
-  for (int s0 = 0; s0 < nv; s0 += ns) {
-    const int sc = std::min(ns, nv - s0);
-    const size_t cols = (size_t)sc * nc, rows = (size_t)(nv - s0) * nc;
-    hipLaunchKernelGGL(k_kmat, dim3(grid_of(npr * cols, 256)), dim3(256), 0, st, dCo, dCv, dLM, n, nc, s0, sc, npr, dK);
-    // Y = X[rows (a, r >= s0)] K   (rows x cols)
-    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)rows, (rocblas_int)cols,
-                      (rocblas_int)npr, &one, dX + (size_t)s0 * nc, (rocblas_int)ncv, dK, (rocblas_int)npr, &zero, dY,
-                      (rocblas_int)rows) != rocblas_status_success)
-      return bad("rocblas_dgemm (second half) failed");
-    flop2 += 2.0 * (double)rows * cols * npr;
-    hipLaunchKernelGGL(k_energy, dim3(G), dim3(kEnergyBlock), 0, st, dY, nc, nv, s0, sc, dec, dev, dPart);
+  if (nspin == 1) {
+    if (int rc = second_half<0>(c, dX[0], sp[0], sp[0], &flop2)) return rc;
+  } else {
+    for (int k = 0; k < 2; ++k)
+      if (sp[k].nc > 1 && sp[k].nv > 1)
+        if (int rc = second_half<1>(c, dX[k], sp[k], sp[k], &flop2)) return rc;
+    if (sp[0].nc > 0 && sp[0].nv > 0 && sp[1].nc > 0 && sp[1].nv > 0)
+      if (int rc = second_half<2>(c, dX[0], sp[0], sp[1], &flop2)) return rc;
   }
-  if (hipEventRecord(e2, st) != hipSuccess) return bad("hipEventRecord failed");
-  std::vector<double> part(G);
-  if (hipMemcpyAsync(part.data(), dPart, G * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipEventRecord(e2, st) != hipSuccess) return fail("hipEventRecord failed");
+  std::vector<double> part(kParts);
+  if (hipMemcpyAsync(part.data(), dPart, kParts * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
-    return bad(std::string("second half-transform failed: ") + hipGetErrorString(hipGetLastError()));
+    return fail(std::string("energy sum failed: ") + hipGetErrorString(hipGetLastError()));
   double e2sum = 0.0;
   for (double v : part) e2sum += v;
   float ms1 = 0.0f, ms2 = 0.0f;
@@ -232,7 +303,10 @@ int sq_mp2_from_pairs(hipStream_t st, int n, int nc, int nv, const double* dCo, 
   out->e2 = e2sum;
   out->half1_ms = ms1;
   out->half2_ms = ms2;
-  out->half1_flop = 2.0 * nc * (double)n * n * npr + 2.0 * nc * (double)nv * n * npr;
+  double f1 = 0.0;
+  for (int k = 0; k < nspin; ++k)
+    f1 += 2.0 * sp[k].nc * (double)n * n * npr + 2.0 * sp[k].nc * (double)sp[k].nv * n * npr;
+  out->half1_flop = f1;
   out->half2_flop = flop2;
   return 0;
 }

@@ -1,8 +1,8 @@
 /*
  * sq_mp2.c - drop-ins for src/mp2.h: mp2_rhf_aqij_Parallel, mp2_rhf_aqij,
- * getFrozenCore_RHF.  The integrals, the four-index transform and the energy
+ * mp2_uhf_aqij, getFrozenCore_RHF.  The integrals, the four-index transform and the energy
  * sum run on the device (sqjk_mp2_rhf); this file counts the frozen core and
- * prints the reference's report around it (src/mp2.c:713-927).
+ * prints the reference's report around it (src/mp2.c:713-927, 2126-2432).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -82,4 +82,43 @@ double mp2_rhf_aqij(int nBasis, int nOcc, const double *e, const double *C, cons
                     const struct Molecule_t *mol, const struct option_t *opt)
 {
 	return mp2_rhf_aqij_Parallel(nBasis, nOcc, e, C, gto, mol, opt);
+}
+
+double mp2_uhf_aqij(int nBasis, int nEA, int nEB, const double *eA, const double *eB, const double *CA,
+                    const double *CB, const struct GTOBasis_t *gto, const struct Molecule_t *mol,
+                    const struct option_t *opt)
+{
+	(void)opt;
+	const int nCore = getFrozenCore_RHF(mol);
+	printf("                                                             \n"
+	       "                                                             \n"
+	       "-------------------------------------------------------------\n"
+	       "-----        UNRESTRICTED MP2 ENERGY CORRECTION         -----\n"
+	       "-------------------------------------------------------------\n");
+	printf("The number of excluded core orbitals:      %4d %4d\n", nCore, nCore);
+	printf("The number of included occupied orbitals:  %4d %4d\n", nEA - nCore, nEB - nCore);
+	printf("The number of included virtual  orbitals:  %4d %4d\n", nBasis - nEA, nBasis - nEB);
+	fflush(stdout);
+	if (nCore > nEA || nCore > nEB) {
+		printf("mp2_uhf_aqij - error more frozen-core orbitals (%d) than occupied orbitals (%d %d)\n", nCore, nEA,
+		       nEB);
+		exit(-1);
+	}
+	sqjk_ctx *ctx = NULL;
+	int rc = sqjk_create(nBasis, gto, NULL, 0, &ctx);
+	sqjk_mp2_info info;
+	if (rc == SQJK_OK) rc = sqjk_mp2_uhf(ctx, nEA, nEB, nCore, eA, eB, CA, CB, MP2_DROPIN_CUTOFF, &info);
+	if (ctx) sqjk_destroy(ctx);
+	if (rc != SQJK_OK) {
+		printf("mp2_uhf_aqij - error %d: %s\n", rc, sqjk_errmsg());
+		exit(-1);
+	}
+	printf("Device memory for (pq|ij) and (ar|ij):       %8.1f MB\n", info.device_bytes * 1.0e-6);
+	printf("\n"
+	       "MP2 Energy Correction is %12.6f Hartree\n"
+	       "                         %12.6f eV\n",
+	       info.e2, HARTREE_TO_EV * info.e2);
+	printf("-------------------------------------------------------------\n");
+	fflush(stdout);
+	return info.e2;
 }

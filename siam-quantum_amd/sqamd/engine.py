@@ -171,6 +171,17 @@ class JKEngine:
                "sqjk_mp2_rhf")
         return {f: getattr(info, f) for f, _ in Mp2Info._fields_}
 
+    def mp2_uhf(self, nea, neb, ncore, eA, eB, CA, CB, cutoff=0.0):
+        """Unrestricted MP2 correlation energy and timings (sqjk_mp2_uhf)."""
+        L = lib()
+        L.sqjk_mp2_uhf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, PD, PD, PD, PD, C.c_double,
+                                   C.POINTER(Mp2Info)]
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eA, eB, CA, CB)]
+        info = Mp2Info()
+        _check(L.sqjk_mp2_uhf(self._h, int(nea), int(neb), int(ncore), *[dptr(x) for x in a], float(cutoff),
+                              C.byref(info)), "sqjk_mp2_uhf")
+        return {f: getattr(info, f) for f, _ in Mp2Info._fields_}
+
     def class_times(self):
         """Kernel ms per class of the last build (profile mode), class_stats order."""
         t = (C.c_double * 21)()
@@ -251,3 +262,15 @@ def frozen_core(mol):
     L = lib()
     L.getFrozenCore_RHF.argtypes = [C.POINTER(Molecule)]
     return L.getFrozenCore_RHF(mol.ptr)
+
+
+def mp2_uhf_aqij(bas, mol, nea, neb, eA, eB, CA, CB, opt):
+    """The mp2_uhf_aqij drop-in: frozen-core unrestricted MP2 energy
+    correction (prints the reference's report)."""
+    from .abi import Molecule
+    L = lib()
+    L.mp2_uhf_aqij.argtypes = [C.c_int, C.c_int, C.c_int, PD, PD, PD, PD, PG, C.POINTER(Molecule),
+                               C.POINTER(Option)]
+    L.mp2_uhf_aqij.restype = C.c_double
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (eA, eB, CA, CB)]
+    return L.mp2_uhf_aqij(bas.n, int(nea), int(neb), *[dptr(x) for x in a], bas.ptr, mol.ptr, C.byref(opt))

@@ -16,7 +16,7 @@ import pytest
 from molecules import g2
 from sqamd import JKEngine, default_option, inputs
 from sqamd.abi import Mol
-from sqamd.engine import frozen_core, mp2_rhf_aqij_parallel
+from sqamd.engine import frozen_core, mp2_rhf_aqij_parallel, mp2_uhf_aqij
 
 pytestmark = pytest.mark.gpu
 CRAWFORD_BOHR = np.array([[0.0, -0.143225816552, 0.0], [1.638036840407, 1.136548822547, 0.0],
@@ -115,5 +115,47 @@ def test_mp2_dropin_vs_oracle(oracle, capfd):
     print(f"drop-in {e2:.14f} reference rules {rules:.14f}")
     assert "MP2 ENERGY CORRECTION" in out
     assert "The number of excluded core orbitals:          %4d" % 2 in out
+    assert f"MP2 Energy Correction is {rules:12.6f} Hartree" in out
+    assert abs(e2 - rules) < 1e-9
+
+
+def _cation(oracle, name, basis="631g_star"):
+    mol, bas = g2(name, basis, 1)
+    ne = mol.n_electrons
+    nea, neb = (ne + 1) // 2, ne // 2
+    _, _, (CA, CB, eA, eB) = oracle.uhf(bas.ptr, bas.n, mol.ptr, nea, neb, restricted=False, scf_conv=1e-10,
+                                        want_orbitals=True)
+    return mol, bas, nea, neb, CA, CB, eA, eB
+
+
+@pytest.mark.parametrize("name", ["h2o_q0m1", "c2h4_q0m1"])
+def test_mp2_uhf_vs_oracle(oracle, name):
+    """Open-shell cations (doublets), same-spin and opposite-spin blocks."""
+    mol, bas, nea, neb, CA, CB, eA, eB = _cation(oracle, name)
+    nc = oracle.mp2_frozen_core(mol.ptr)
+    r = JKEngine(bas.ptr, bas.n).mp2_uhf(nea, neb, nc, eA, eB, CA, CB, 0.0)
+    x = oracle.mp2_uhf(bas.ptr, bas.n, nea, neb, nc, eA, eB, CA, CB, 0.0)
+    print(f"{name}+: device {r['e2']:.14f} oracle {x:.14f} d={r['e2'] - x:.1e}")
+    assert abs(r["e2"] - x) < 1e-10
+
+
+def test_mp2_uhf_closed_shell_equals_rhf(oracle):
+    mol, bas = g2("c2h4_q0m1", "631g_star")
+    Cm, e = _rhf(oracle, mol, bas)
+    jk = JKEngine(bas.ptr, bas.n)
+    r = jk.mp2_rhf(8, 2, e, Cm, 0.0)["e2"]
+    u = jk.mp2_uhf(8, 8, 2, e, e, Cm, Cm, 0.0)["e2"]
+    print(f"RMP2 {r:.15f} UMP2 {u:.15f}")
+    assert abs(r - u) < 1e-11
+
+
+def test_mp2_uhf_dropin_vs_oracle(oracle, capfd):
+    mol, bas, nea, neb, CA, CB, eA, eB = _cation(oracle, "h2o_q0m1")
+    e2 = mp2_uhf_aqij(bas, mol, nea, neb, eA, eB, CA, CB, default_option(restricted=0))
+    out = capfd.readouterr().out
+    rules = oracle.mp2_uhf(bas.ptr, bas.n, nea, neb, 1, eA, eB, CA, CB, 1e-8)
+    print(out)
+    assert "UNRESTRICTED MP2 ENERGY CORRECTION" in out
+    assert "The number of included occupied orbitals:  %4d %4d" % (nea - 1, neb - 1) in out
     assert f"MP2 Energy Correction is {rules:12.6f} Hartree" in out
     assert abs(e2 - rules) < 1e-9

@@ -8,6 +8,10 @@ src/mp2.c:581-712 and src/matrix.c:4797-5073) pinned three ways:
   transform, Szabo-Ostlund eq. 6.74), exact and with a frozen core;
 - the compiled reference's own mp2_rhf_aqij_Parallel (oracle/_ref) at its
   MP2_SCHWARZ_CUTOFF rules (1e-8), water and ethylene in 6-31G*.
+
+The unrestricted MP2 (src/mp2.c:2126-2432) is pinned the same way: against
+mp2_uhf_aqij of the compiled reference (the water cation), the spin-orbital
+textbook sum, and the closed-shell identity E_UMP2(C, C) = E_RMP2(C).
 """
 import os
 
@@ -116,3 +120,56 @@ def test_mp2_oracle_vs_reference(oracle, reference_mp2, name):
     print(f"{name}: rules {rules:.15f} reference {ref:.15f} exact {exact:.15f}")
     assert abs(rules - ref) < 1e-14
     assert abs(exact - ref) < 1e-11
+
+
+def _water_cation(oracle):
+    Z, X = inputs.water_c1()
+    mol = inputs.mol_from_angstrom(Z, X, 1)
+    bas = inputs.make_basis(mol, "631g_star")
+    _, _, (CA, CB, eA, eB) = oracle.uhf(bas.ptr, bas.n, mol.ptr, 5, 4, restricted=False, scf_conv=1e-10,
+                                        want_orbitals=True)
+    return mol, bas, CA, CB, eA, eB
+
+
+def test_mp2_uhf_closed_shell_equals_rhf(oracle):
+    mol, bas = g2("h2o_q0m1", "631g_star")
+    _, Cm, e = _rhf(oracle, mol, bas, 1e-10)
+    r = oracle.mp2_rhf(bas.ptr, bas.n, 5, 1, e, Cm, 0.0)
+    u = oracle.mp2_uhf(bas.ptr, bas.n, 5, 5, 1, e, e, Cm, Cm, 0.0)
+    assert abs(r - u) < 1e-14
+
+
+def test_mp2_uhf_oracle_vs_reference(oracle, reference_mp2):
+    mol, bas, CA, CB, eA, eB = _water_cation(oracle)
+    rules = oracle.mp2_uhf(bas.ptr, bas.n, 5, 4, 1, eA, eB, CA, CB, 1e-8)
+    ref = reference_mp2.mp2_uhf(bas.ptr, bas.n, mol.ptr, 5, 4, eA, eB, CA, CB)
+    print(f"rules {rules:.15f} reference {ref:.15f}")
+    assert abs(rules - ref) < 1e-14
+
+
+def test_mp2_uhf_vs_textbook_spin_orbitals(oracle):
+    """E2 = 1/4 sum |<ij||ab>|^2 / D per spin plus the alpha-beta sum, from
+    the full AO tensor."""
+    mol, bas, CA, CB, eA, eB = _water_cation(oracle)
+    n = bas.n
+    V = oracle.eri_pairs(bas.ptr, n)
+    idx = np.array([[max(p, q) * (max(p, q) + 1) // 2 + min(p, q) for q in range(n)] for p in range(n)])
+    T = V[idx[:, :, None, None], idx[None, None, :, :]]
+
+    def mo(c1, c2, c3, c4):
+        return np.einsum("pqrs,pa,qi,rb,sj->aibj", T, c1, c2, c3, c4, optimize=True)
+
+    def denom(eo1, ev1, eo2, ev2):
+        return eo1[:, None, None, None] - ev1[None, :, None, None] + eo2[None, None, :, None] - ev2[None, None, None, :]
+
+    Ca, Cb = CA.T, CB.T
+    oa, va, ob, vb = Ca[:, 1:5], Ca[:, 5:], Cb[:, 1:4], Cb[:, 4:]
+    eo_a, ev_a, eo_b, ev_b = eA[1:5], eA[5:], eB[1:4], eB[4:]
+    total = 0.0
+    for o_, v_, eo, ev in ((oa, va, eo_a, ev_a), (ob, vb, eo_b, ev_b)):
+        g = mo(o_, v_, o_, v_)
+        total += 0.25 * np.sum((g - g.transpose(0, 3, 2, 1)) ** 2 / denom(eo, ev, eo, ev))
+    total += np.sum(mo(oa, va, ob, vb) ** 2 / denom(eo_a, ev_a, eo_b, ev_b))
+    x = oracle.mp2_uhf(bas.ptr, n, 5, 4, 1, eA, eB, CA, CB, 0.0)
+    print(f"oracle {x:.15f} textbook {total:.15f}")
+    assert abs(x - total) < 1e-13
