@@ -159,3 +159,15 @@ def test_mp2_uhf_dropin_vs_oracle(oracle, capfd):
     assert "The number of included occupied orbitals:  %4d %4d" % (nea - 1, neb - 1) in out
     assert f"MP2 Energy Correction is {rules:12.6f} Hartree" in out
     assert abs(e2 - rules) < 1e-9
+
+
+def test_mp2_rejects_a_pair_matrix_beyond_device_memory():
+    """(H2O)64/6-31G* (1216 functions) would need 4.4 TB for the pair matrix:
+    SQJK_ENOMEM with the sizes, before any allocation."""
+    Z, X = inputs.water_cluster(64, 11)
+    mol = inputs.mol_from_angstrom(Z, X)
+    bas = inputs.make_basis(mol, "631g_star")
+    n = bas.n
+    jk = JKEngine(bas.ptr, n)
+    with pytest.raises(RuntimeError, match="MiB of device memory"):
+        jk.mp2_rhf(320, 64, np.zeros(n), np.eye(n), 1e-10)
