@@ -1,6 +1,10 @@
 """Writes tests/golden/scf_w16.json: the converged RHF/6-31G* energy of the
 (H2O)16 cluster (seed 1601, 304 functions) on the reference's own two-electron
-code.
+code, and the frozen-core MP2 energy of those orbitals three ways: by the
+reference's own mp2_rhf_aqij_Parallel (src/mp2.c:713-927,
+oracle/_ref/libsqref_mp2.so, one process, one cycle: maxMem 9000 MB, so no
+multipole substitution), and by the oracle's restatement of it with the
+reference's 1e-8 screening rules and with none (the exact energy).
 
 J and K come from the compiled reference (oracle/_ref/libsqref.so,
 GTO_JK_Matrix_Quartet_Parallel of src/matrix.c:3773-3986) at near-exact
@@ -18,7 +22,10 @@ DIIS on the Fock matrix, generalized eigenproblem by LAPACK through scipy)
 converged to |dE| < 1e-11 Eh and a DIIS error below 1e-8: at convergence the
 energy does not depend on how the iteration got there, so it pins the
 device SCF's energy (its own cut-offs stay far below the 1e-8 Eh bar of the
-test that reads this file).
+test that reads this file).  The MP2 energy is a function of the orbitals
+that is stationary in neither; at this convergence it is fixed to about
+1e-10 Eh, and the reference's own 1e-8 screening rules (MP2_SCHWARZ_CUTOFF)
+move it by less than 1e-9 Eh on the molecules the oracle tests cover.
 
 Usage (this container, a few minutes on 8 cores):
   python tests/golden/make_scf_fixture.py
@@ -63,6 +70,17 @@ def _jk_slice(args):
     return J, KA
 
 
+def _mp2_job(args):
+    kind, nocc, eps, Cr = args
+    from molecules import cluster
+    from oracle import Oracle, ReferenceMP2
+    mol, bas = cluster(16, 1601)
+    if kind == "reference":
+        return ReferenceMP2().mp2(bas.ptr, bas.n, mol.ptr, nocc, eps, Cr, max_mem=9000)
+    o = Oracle()
+    return o.mp2_rhf(bas.ptr, bas.n, nocc, o.mp2_frozen_core(mol.ptr), eps, Cr, 1e-8 if kind == "rules" else 0.0)
+
+
 def main():
     import scipy.linalg as sl
     from molecules import cluster
@@ -81,7 +99,7 @@ def main():
     xyz = np.array([[m.x[i], m.y[i], m.z[i]] for i in range(m.nAtom)])
     Z = np.array([m.Z[i] for i in range(m.nAtom)], dtype=float)
     Enn = sum(Z[i] * Z[j] / np.linalg.norm(xyz[i] - xyz[j]) for i in range(len(Z)) for j in range(i))
-    _, Cm = sl.eigh(H, S)
+    eps, Cm = sl.eigh(H, S)
     P = Cm[:, :nocc] @ Cm[:, :nocc].T  # alpha density
     E, Fs, Es, it = None, [], [], 0
     pools = [mp.Pool(1, initializer=_init) for _ in range(NPROC)]  # one process per childID
@@ -108,7 +126,7 @@ def main():
             rhs[k] = -1.0
             c = np.linalg.solve(B, rhs)[:k]
             Fd = sum(ci * Fi for ci, Fi in zip(c, Fs))
-            _, Cm = sl.eigh(Fd, S)
+            eps, Cm = sl.eigh(Fd, S)
             P = Cm[:, :nocc] @ Cm[:, :nocc].T
             emax = float(np.abs(err).max())
             print(f"iter {it}: E = {E:.12f}  max|FPS-SPF| = {emax:.2e}  {time.time() - t0:.0f} s", flush=True)
@@ -119,10 +137,20 @@ def main():
     finally:
         for p in pools:
             p.terminate()
+    t_scf = time.time() - t0
+    from oracle import ReferenceMP2
+    ncore = ReferenceMP2().L.getFrozenCore_RHF(mol.ptr)
+    Cr = np.ascontiguousarray(Cm.T)
+    with mp.Pool(3) as pool:
+        e2, e2_rules, e2_exact = pool.map(_mp2_job, [(k, nocc, eps, Cr) for k in ("reference", "rules", "exact")])
     out = {"workload": "(H2O)16 cluster, seed 1601, RHF/6-31G*", "n": n, "electrons": ne,
            "energy": E, "iterations": it, "converged": "|dE| < 1e-11, max|FPS-SPF| < 1e-8",
            "two_electron": "reference GTO_JK_Matrix_Quartet_Parallel, fixedCutoff 1e-14, SCFCutoff 1e-13",
-           "seconds": round(time.time() - t0, 1)}
+           "seconds": round(t_scf, 1),
+           "mp2_energy": e2, "mp2_frozen_core": ncore,
+           "mp2": "reference mp2_rhf_aqij_Parallel on the final orbitals, nCPU 1, one cycle",
+           "mp2_energy_oracle_rules": e2_rules, "mp2_energy_oracle_exact": e2_exact,
+           "mp2_seconds": round(time.time() - t0 - t_scf, 1)}
     with open(os.path.join(HERE, "scf_w16.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(out)
