@@ -22,12 +22,13 @@ DIIS on the Fock matrix, generalized eigenproblem by LAPACK through scipy)
 converged to |dE| < 1e-11 Eh and a DIIS error below 1e-8: at convergence the
 energy does not depend on how the iteration got there, so it pins the
 device SCF's energy (its own cut-offs stay far below the 1e-8 Eh bar of the
-test that reads this file).  The MP2 energy is a function of the orbitals
-that is stationary in neither; at this convergence it is fixed to about
-1e-10 Eh, and the reference's own 1e-8 screening rules (MP2_SCHWARZ_CUTOFF)
-move it by less than 1e-9 Eh on the molecules the oracle tests cover.
+test that reads this file).  The MP2 energy is not stationary in the
+orbitals; at this convergence it is fixed to a few 1e-10 Eh.  The
+reference's own 1e-8 screening rules (MP2_SCHWARZ_CUTOFF) move it by
+1.2e-8 Eh here, which is why the exact energy is written beside it.
 
-Usage (this container, a few minutes on 8 cores):
+Usage (this container: the SCF about 2 minutes on 8 cores, the three MP2
+energies about 35 minutes, the exact one on the oracle the longest):
   python tests/golden/make_scf_fixture.py
 """
 import ctypes as C
