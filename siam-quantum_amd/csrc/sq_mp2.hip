@@ -34,13 +34,41 @@ __device__ __forceinline__ size_t pair_at(int p, int q) {
   return p >= q ? (size_t)p * (p + 1) / 2 + q : (size_t)q * (q + 1) / 2 + p;
 }
 
-// M[b][p][q] = V[(j0 + b) * ld + pair(p, q)], b < nb
-__global__ void k_unpack(const double* __restrict__ V, size_t ld, long long j0, int nb, int n, double* __restrict__ M) {
-  const size_t nn = (size_t)n * n, total = nn * nb;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t b = e / nn, r = e - b * nn;
-    const int p = (int)(r / n), q = (int)(r - (size_t)p * n);
-    M[e] = V[(j0 + b) * ld + pair_at(p, q)];
+// M[b][p][q] = V[(j0 + b) * ld + pair(p, q)], b < nb.  A workgroup (32 x 8
+// lanes) takes one 32 x 32 tile p in [P0, P0+32), q in [Q0, Q0+32), Q0 <= P0,
+// of one pair row: it reads the row's packed lower triangle (contiguous in q
+// for each p), writes M[p][q] row by row and, through an LDS transpose, the
+// mirrored tile M[q][p] row by row as well.
+constexpr int kTile = 32;
+__global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ V, size_t ld, long long j0, int n,
+                                                double* __restrict__ M) {
+  __shared__ double t[kTile][kTile + 1];
+  const int nt = (n + kTile - 1) / kTile;
+  const int b = blockIdx.y;
+  // tile index -> (ti >= tj) over the lower triangle of tiles
+  int ti = (int)((sqrt(8.0 * blockIdx.x + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= (int)blockIdx.x) ++ti;
+  while (ti * (ti + 1) / 2 > (int)blockIdx.x) --ti;
+  const int tj = (int)blockIdx.x - ti * (ti + 1) / 2;
+  if (ti >= nt) return;
+  const int P0 = ti * kTile, Q0 = tj * kTile;
+  const double* row = V + (size_t)(j0 + b) * ld;
+  double* Mb = M + (size_t)b * n * n;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < kTile; i += 8) {
+    const int p = P0 + i, q = Q0 + tx;
+    double v = 0.0;
+    if (p < n && q < n && q <= p) {
+      v = row[(size_t)p * (p + 1) / 2 + q];
+      Mb[(size_t)p * n + q] = v;
+    }
+    t[i][tx] = v;
+  }
+  __syncthreads();
+  // mirrored tile: M[q][p] for q in [Q0, Q0+32) (rows), p in [P0, P0+32)
+  for (int i = ty; i < kTile; i += 8) {
+    const int q = Q0 + i, p = P0 + tx;
+    if (p < n && q < n && q < p) Mb[(size_t)q * n + p] = t[tx][i];
   }
 }
 
@@ -159,15 +187,16 @@ int first_half(Ctx& c, const double* dV, const SqMp2Spin& sp, double* dX) {
   if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return c.bad("hipMemGetInfo failed");
   const size_t per_pair = (size_t)n * n * 8 + (size_t)nc * n * 8;
   const size_t blk_budget = std::min<size_t>(freeb / 4, (size_t)2 << 30);
-  const int nb = (int)std::max<size_t>(1, std::min<size_t>({npr, blk_budget / per_pair, (size_t)INT32_MAX / n}));
+  // (pairs per block: the unpack grid holds them in its y dimension)
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>({npr, blk_budget / per_pair, (size_t)INT32_MAX / n, (size_t)65535}));
   double *dM = nullptr, *dT = nullptr;
   Bufs blk;
   if (!blk.get(&dM, (size_t)nb * n * n) || !blk.get(&dT, (size_t)nc * nb * n))
     return c.bad("device allocation of the unpack blocks failed");
   for (size_t j0 = 0; j0 < npr; j0 += nb) {
     const int bc = (int)std::min<size_t>(nb, npr - j0);
-    hipLaunchKernelGGL(k_unpack, dim3(grid_of((size_t)bc * n * n, 256)), dim3(256), 0, c.st, dV, npr, (long long)j0, bc,
-                       n, dM);
+    const int nt = (n + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_unpack, dim3(nt * (nt + 1) / 2, bc), dim3(256), 0, c.st, dV, npr, (long long)j0, n, dM);
     // T = Co^T [M_0 ... M_bc-1]   (nc x bc n)
     if (rocblas_dgemm(c.h, rocblas_operation_transpose, rocblas_operation_none, nc, bc * n, n, &one, sp.dCo, n, dM, n,
                       &zero, dT, nc) != rocblas_status_success)
