@@ -20,7 +20,8 @@
  *   J/K digestion       src/matrix.c:2934-3396 (six PQIJ cases)
  *   J/K driver          src/matrix.c:3703-3986
  *   SCF                 src/uhf.c:48-315, 1171-1783; src/conv.c:606-844
- *   MP2                 src/mp2.c:29-67, 581-712; src/matrix.c:4797-5073
+ *   MP2                 src/mp2.c:29-67, 581-712, 2126-2432;
+ *                       src/matrix.c:4797-5073
  * How each part is pinned (tests/test_oracle.py unless named):
  *   Boys                mpmath quadrature; recursion invariant; bit-identical
  *                       to the compiled reference (oracle/_ref)
@@ -34,10 +35,11 @@
  *                       monopole substitution)
  *   readers             tests/test_host_c.py
  *   SCF                 GAMESS and the reference's own log energies
- *   MP2                 the compiled reference's mp2_rhf_aqij_Parallel at its
- *                       own cut-off; the textbook full-tensor sum over MO
- *                       integrals; the published H2O/STO-3G MP2 energy
- *                       (tests/test_oracle_mp2.py)
+ *   MP2                 the compiled reference's mp2_rhf_aqij_Parallel and
+ *                       mp2_uhf_aqij at their own cut-off; the textbook
+ *                       full-tensor sums over MO and spin orbitals; the
+ *                       published H2O/STO-3G MP2 energy; the closed-shell
+ *                       identity UMP2 = RMP2 (tests/test_oracle_mp2.py)
  * Parity unpinned: eigenvector phases and the DIIS path of the SCF (only
  * converged energies fix them).
  */
