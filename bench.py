@@ -83,10 +83,18 @@ def mp2_bench(name, steps):
     """Frozen-core RHF MP2 energy on the device (sqjk_mp2_rhf, Schwarz cut-off
     1e-10) on the orbitals of a converged SCF (uhf() drop-in, SCFConv 1e-8):
     (H2O)16/6-31G*, or C60/3-21G, the molecule of the reference's published
-    MP2 timing (47 min on two i7 cores, docs/FAQ.md:121-129)."""
+    MP2 timing (47 min on two i7 cores, docs/FAQ.md:121-129); "w16_cation":
+    unrestricted MP2 (sqjk_mp2_uhf) of the (H2O)16 cation doublet on its
+    UHF orbitals."""
     from sqamd import JKEngine, inputs
     from sqamd.engine import frozen_core
-    if name == "c60_321g":
+    uhf = name == "w16_cation"
+    if uhf:
+        mol0, _, d = molecule("w16")
+        mol = inputs.mol_from_angstrom(mol0.Z, np.stack([mol0.x, mol0.y, mol0.z], 1) / inputs.ANGSTROM2BOHR, 1)
+        bas = inputs.make_basis(mol, "631g_star")
+        desc = f"{d} cation doublet UHF/6-31G*"
+    elif name == "c60_321g":
         Z, X = inputs.c60()
         mol = inputs.mol_from_angstrom(Z, X)
         bas = inputs.make_basis(mol, "321g")
@@ -96,14 +104,29 @@ def mp2_bench(name, steps):
         bas = inputs.make_basis(mol, "631g_star")
         desc = f"{d} RHF/6-31G*"
     orbs = {}
-    E, _, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-8)
-    nocc, ncore = mol.n_electrons // 2, frozen_core(mol)
+    ne, ncore = mol.n_electrons, frozen_core(mol)
+    if uhf:
+        from sqamd import default_option, dptr
+        n = bas.n
+        nea, neb = (ne + 1) // 2, ne // 2
+        CA, CB, eA, eB = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+        opt = default_option(SCFConv=1e-8, restricted=0, multiplicity=2)
+        t0 = time.perf_counter()
+        E = uhf_fn().uhf(n, bas.ptr, mol.ptr, nea, neb, dptr(CA), dptr(CB), dptr(eA), dptr(eB), C.byref(opt))
+        t_scf = time.perf_counter() - t0
+        nocc = nea
+    else:
+        E, _, t_scf = scf_once(mol, bas, orbs, SCFConv=1e-8)
+        nocc = ne // 2
     eng = JKEngine(bas.ptr, bas.n)
     runs = []
     for _ in range(max(1, steps)):
         t0 = time.perf_counter()
         try:
-            r = eng.mp2_rhf(nocc, ncore, orbs["eA"], orbs["CA"], 1e-10)
+            if uhf:
+                r = eng.mp2_uhf(nea, neb, ncore, eA, eB, CA, CB, 1e-10)
+            else:
+                r = eng.mp2_rhf(nocc, ncore, orbs["eA"], orbs["CA"], 1e-10)
         except RuntimeError as e:
             eng.close()
             return {"workload": desc, "error": str(e)}
@@ -113,6 +136,7 @@ def mp2_bench(name, steps):
     r = min(runs, key=lambda x: x["seconds"])
     out = {"workload": f"{desc} frozen-core MP2 on the converged SCF, FP64, Schwarz 1e-10",
            "basis_functions": bas.n, "correlated_occupied": nocc - ncore, "virtual": bas.n - nocc,
+           **({"correlated_occupied_beta": neb - ncore, "virtual_beta": bas.n - neb} if uhf else {}),
            "scf_energy": E, "scf_seconds": t_scf, "mp2_energy": r["e2"], "seconds": r["seconds"],
            "runs": len(runs), "integral_store_ms": r["store_ms"], "first_half_ms": r["half1_ms"],
            "second_half_ms": r["half2_ms"], "second_half_tflops": r["half2_flop"] / (r["half2_ms"] * 1e-3) / 1e12,
@@ -357,7 +381,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--forces-only", action="store_true",
                     help="only the C60 SCF and its forces (a JSON line with the forces timings), for profiling")
-    ap.add_argument("--mp2-only", choices=["w16", "c60_321g"], default=None,
+    ap.add_argument("--mp2-only", choices=["w16", "w16_cation", "c60_321g"], default=None,
                     help="only the MP2 benchmark of this workload")
     ap.add_argument("--no-mp2-c60", action="store_true", help="leave out the C60/3-21G MP2 extra")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -510,6 +534,7 @@ def main():
             out["scf"] = scf_solves(a.scf_workload, a.scf_solves, a.mixed_threshold)
         print("bench: MP2 (H2O)16/6-31G*", file=sys.stderr, flush=True)
         out["mp2"] = mp2_bench("w16", 2)
+        out["mp2_uhf"] = mp2_bench("w16_cation", 2)
         if not a.no_mp2_c60:
             print("bench: MP2 C60/3-21G", file=sys.stderr, flush=True)
             out["mp2_c60"] = mp2_bench("c60_321g", 1)
