@@ -162,17 +162,24 @@ void uhf_force(int nBasis, struct GTOBasis_t *gto, struct Molecule_t *mol, int n
 	uhf_getDMatrix(n, nEA, CA, PA);
 	uhf_getDMatrix(n, nEB, CB, PB);
 	for (size_t e = 0; e < nn; e++) PT[e] = PA[e] + PB[e];
-	/* energy-weighted density QT = sum_i e_i C_i C_i^T over both spins */
-	for (int i = 0; i < nEA; i++)
-		for (int p = 0; p < n; p++) {
-			const double cp = eA[i] * CA[(size_t)i * n + p];
-			for (int q = 0; q < n; q++) QT[(size_t)p * n + q] += cp * CA[(size_t)i * n + q];
+	/* energy-weighted density QT = sum_i e_i C_i C_i^T over both spins, as
+	 * rank-one updates over blocks of 16 rows (alpha orbitals, then beta, in
+	 * increasing i for every element) */
+	for (int p0 = 0; p0 < n; p0 += 16) {
+		const int p1 = p0 + 16 < n ? p0 + 16 : n;
+		for (int s = 0; s < 2; s++) {
+			const int no = s ? nEB : nEA;
+			const double *Cs = s ? CB : CA, *es = s ? eB : eA;
+			for (int i = 0; i < no; i++) {
+				const double *c = Cs + (size_t)i * n;
+				for (int p = p0; p < p1; p++) {
+					const double cp = es[i] * c[p];
+					double *row = QT + (size_t)p * n;
+					for (int q = 0; q < n; q++) row[q] += cp * c[q];
+				}
+			}
 		}
-	for (int i = 0; i < nEB; i++)
-		for (int p = 0; p < n; p++) {
-			const double cp = eB[i] * CB[(size_t)i * n + p];
-			for (int q = 0; q < n; q++) QT[(size_t)p * n + q] += cp * CB[(size_t)i * n + q];
-		}
+	}
 	/* nuclear repulsion (src/grad.c:43-66) */
 	for (int a = 0; a < na; a++)
 		for (int b = 0; b < a; b++) {

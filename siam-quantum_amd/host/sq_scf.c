@@ -34,14 +34,24 @@
 		}                                                        \
 	} while (0)
 
+/* P = sum_{k < nOcc} C_k C_k^T as rank-one updates over blocks of 16 rows
+ * (the block stays in cache while the orbitals stream past; each element
+ * sums its terms in increasing k, as the plain triple loop does) */
 void uhf_getDMatrix(int nBasis, int nOcc, double *C, double *P)
 {
-	for (int i = 0; i < nBasis; i++)
-		for (int j = 0; j < nBasis; j++) {
-			double s = 0.0;
-			for (int k = 0; k < nOcc; k++) s += C[k * nBasis + i] * C[k * nBasis + j];
-			P[i * nBasis + j] = s;
+	const int n = nBasis;
+	for (int i0 = 0; i0 < n; i0 += 16) {
+		const int i1 = i0 + 16 < n ? i0 + 16 : n;
+		memset(P + (size_t)i0 * n, 0, sizeof(double) * (size_t)(i1 - i0) * n);
+		for (int k = 0; k < nOcc; k++) {
+			const double *c = C + (size_t)k * n;
+			for (int i = i0; i < i1; i++) {
+				const double ci = c[i];
+				double *row = P + (size_t)i * n;
+				for (int j = 0; j < n; j++) row[j] += ci * c[j];
+			}
 		}
+	}
 }
 
 void rhf_getDMatrix(int nBasis, int nOcc, double *C, double *P)
